@@ -1,0 +1,148 @@
+/*
+ * xtn.h -- C ABI of the MI355X XTNeighbor-streaming engine (libxtn.so).
+ *
+ * The reference has no exported library: its algorithm entry is
+ *     void xtn_perform(XTNArgs args, SeqArray* seqArr, SeqInfo* seqInfo,
+ *                      void callback(XTNOutput))          (xtn.cu:267, xtn.h:124)
+ * compiled into the CLI's translation unit. This header is the boundary a
+ * caller (the xt_neighbor CLI, a cgo/ctypes binding) binds instead. Each
+ * declaration names the reference interface it replaces. All types are plain
+ * C; no HIP or torch types cross the boundary. Buffers passed in are host
+ * memory owned by the caller; buffers handed to a callback are owned by the
+ * library and valid only during that call (xtn.cu:458-459, 478-479).
+ *
+ * Every function returns an int status (XTN_OK on success). The library never
+ * calls exit(); xtn_last_error() describes the most recent failure of a handle.
+ * A handle is not safe for concurrent use; distinct handles are independent.
+ */
+#ifndef XTN_H
+#define XTN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* distance measures: LEVENSHTEIN / HAMMING (xtn.h:12-14) */
+#define XTN_LEVENSHTEIN 0
+#define XTN_HAMMING 1
+
+/* status codes. XTN_OK, XTN_ERROR and XTN_EXIT keep the values of the
+ * reference's ReturnCode {SUCCESS, ERROR, EXIT} (xtn.h:93). */
+enum {
+	XTN_OK = 0,
+	XTN_ERROR = 1,         /* invalid input data (bad residue, length mismatch) */
+	XTN_EXIT = 2,          /* reserved: the CLI's help/version exit */
+	XTN_ERR_ARG = 3,       /* invalid argument (NULL pointer, distance < 1, ...) */
+	XTN_ERR_HIP = 4,       /* HIP runtime failure */
+	XTN_ERR_NOMEM = 5,     /* device or pinned host allocation failed */
+	XTN_ERR_CAPACITY = 6,  /* workload exceeds a documented limit */
+	XTN_ERR_STATE = 7      /* call order violated (e.g. run before set_sequences) */
+};
+
+/* Int2 (xtn.h:30-36): a pair of 0-based input row indexes, x < y. */
+typedef struct {
+	int32_t x, y;
+} xtn_int2;
+
+/* SeqInfo (xtn.h:41-45): per input row, its frequency (duplicate_count),
+ * repertoire index and original row index. */
+typedef struct {
+	int32_t frequency, repertoire, original_index;
+} xtn_seq_info;
+
+/* XTNOutput (xtn.h:83-88). NN mode fills index_pairs + pairwise_distances
+ * (ascending (x, y), possibly over several callbacks). Overlap mode fills
+ * index_pairs (repertoire pairs, x <= y) + pairwise_frequencies, in exactly
+ * one callback. */
+typedef struct {
+	const xtn_int2* index_pairs;
+	const int8_t* pairwise_distances;
+	const uint64_t* pairwise_frequencies;
+	int64_t len;
+} xtn_output;
+
+/* the reference's `void callback(XTNOutput)` (xtn.cu:267) plus a user pointer */
+typedef void (*xtn_callback)(void* user, const xtn_output* out);
+
+/* XTNArgs (xtn.h:50-60): distance, measure, verbose. The engine knobs below
+ * replace the reference's MemoryContext budgets (xtn.h:65-78), which it sized
+ * from cudaMemGetInfo and sysinfo (xtn.cu:35-184). */
+typedef struct {
+	int32_t distance;              /* -d, >= 1 */
+	int32_t measure;               /* -m: XTN_LEVENSHTEIN or XTN_HAMMING */
+	int32_t verbose;               /* -V: stage timings on stdout */
+	int32_t device;                /* HIP device ordinal */
+	int64_t max_records_per_chunk; /* 0: size key-space chunks from free HBM;
+	                                  > 0: cap deletion records per chunk
+	                                  (forces the multi-chunk path) */
+	int64_t output_chunk_pairs;    /* NN candidate pairs verified per output
+	                                  chunk (0: default); each chunk is one
+	                                  callback, copied on a side stream */
+	int32_t profile;               /* 1: time every kernel launch with HIP
+	                                  events (see xtn_kernel_profile) */
+	int32_t reserved;
+} xtn_config;
+
+typedef struct xtn_handle xtn_handle;
+
+/* counters of the last run (the reference prints the same quantities in
+ * verbose mode: xtn.cu:336, 378, 430, 468, 472-473) */
+typedef struct {
+	int64_t n_sequences;   /* rows searched (unique rows in overlap mode) */
+	int64_t n_variants;    /* deletion records generated (V) */
+	int64_t n_chunks;      /* key-space chunks */
+	int64_t n_runs;        /* key runs holding >= 2 records */
+	int64_t n_candidates;  /* candidate pairs i != j emitted from runs */
+	int64_t n_unique;      /* unique candidate pairs verified (U) */
+	int64_t n_output;      /* pairs (NN) or repertoire entries (overlap) out */
+	double ms_total;       /* device pipeline wall time */
+	double ms_generate, ms_sort, ms_pairs, ms_verify, ms_output;
+} xtn_stats;
+
+/* fills *cfg with defaults: distance 1, Levenshtein, device 0 */
+void xtn_config_init(xtn_config* cfg);
+
+/* replaces the implicit CUDA context set-up of main() (cli.cu:227-277) */
+int xtn_open(xtn_handle** out, const xtn_config* cfg);
+void xtn_close(xtn_handle* h);
+int xtn_set_config(xtn_handle* h, const xtn_config* cfg);
+
+/* replaces SeqArray::toDevice (codec.cu:211-218): n sequences, chars
+ * concatenated without separators, offsets[n+1] (offsets[0] = 0). Residues
+ * must be ACDEFGHIKLMNPQRSTVWY (codec.cu:25-38); anything else -> XTN_ERROR.
+ * The library keeps a device-resident copy until the next call or close. */
+int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets, int32_t n);
+
+/* replaces xtn_perform (xtn.cu:267). info == NULL: nearest-neighbour mode,
+ * callback invoked once per output chunk in ascending (x, y) order.
+ * info != NULL (n entries, original_index ignored and taken as the row):
+ * repertoire-overlap mode, callback invoked exactly once. */
+int xtn_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, void* user);
+
+/* the NN device pipeline without the host hand-off: device-resident input to
+ * device-resident sorted (x, y, distance) triples. *out_len receives the
+ * output length. Used for device-side timing. */
+int xtn_run_nn_resident(xtn_handle* h, int64_t* out_len);
+/* copies up to cap triples of the last xtn_run_nn_resident to host memory */
+int xtn_copy_nn_output(xtn_handle* h, xtn_int2* pairs, int8_t* distances, int64_t cap);
+
+int xtn_get_stats(xtn_handle* h, xtn_stats* out);
+
+/* profile == 1 only: launches, summed HIP-event duration (ms) and summed
+ * algorithmic HBM bytes of one kernel over the runs since the last reset. */
+int xtn_kernel_profile(xtn_handle* h, const char* kernel, int64_t* launches, double* ms,
+                       double* bytes);
+int xtn_kernel_profile_reset(xtn_handle* h);
+
+const char* xtn_last_error(xtn_handle* h);
+const char* xtn_status_string(int status);
+/* library version string ("2.0.0-mi355x") */
+const char* xtn_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,156 @@
+"""CPU-side checks of the boundary: libxtn.so loads and exports every
+symbol include/xtn.h declares; the xt_neighbor CLI's argument and input
+parsing paths (which finish before the GPU is touched) reproduce the
+reference's messages and exit codes (cli.cu:29-193, airr.cu:116-151)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import xtn
+from conftest import GOLDEN, REPO
+
+HEADER = os.path.join(REPO, "include", "xtn.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(xtn_[a-z_]+)\s*\(", text)) - {"xtn_callback"})
+
+
+def test_library_exports_declared_symbols():
+    lib = xtn.load_library()
+    names = declared_functions()
+    assert set(names) == set(xtn.EXPORTED)
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_version_status_and_defaults():
+    lib = xtn.load_library()
+    assert lib.xtn_version() == b"2.0.0-mi355x"
+    assert lib.xtn_status_string(0) == b"ok"
+    assert lib.xtn_status_string(6) == b"capacity exceeded"
+    cfg = xtn.Config()
+    lib.xtn_config_init(ctypes.byref(cfg))
+    assert (cfg.distance, cfg.measure, cfg.device, cfg.max_records_per_chunk) == (1, 0, 0, 0)
+
+
+def test_open_rejects_bad_config():
+    lib = xtn.load_library()
+    cfg = xtn.Config()
+    lib.xtn_config_init(ctypes.byref(cfg))
+    cfg.distance = 0
+    h = ctypes.c_void_p()
+    assert lib.xtn_open(ctypes.byref(h), ctypes.byref(cfg)) == 3
+    cfg.distance = 1
+    cfg.measure = 5
+    assert lib.xtn_open(ctypes.byref(h), ctypes.byref(cfg)) == 3
+
+
+def run_cli(args, **kw):
+    env = dict(os.environ, LC_ALL="C")
+    return subprocess.run([xtn.CLI_PATH] + args, capture_output=True, env=env, timeout=60, **kw)
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_bytes(text.encode() if isinstance(text, str) else text)
+    return str(p)
+
+
+def test_cli_version_help():
+    r = run_cli(["-v"])
+    assert (r.returncode, r.stdout, r.stderr) == (2, b"2.0.0\n", b"")
+    r = run_cli(["--help"])
+    assert r.returncode == 2
+    assert r.stdout.startswith(b"xt_neighbor: perform either nearest neighbor search")
+    assert b"-N or --info-length [number] (required)" in r.stdout
+    # -v wins over a later unknown flag, as flags are read left to right
+    assert run_cli(["-v", "--bogus"]).returncode == 2
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--bogus"], "unknown option"),
+    (["-n", "5"], "missing path for seq"),
+    (["-i", "x"], "missing length for seq"),
+    (["-i", "x", "-n", "0"], "invalid input length"),
+    (["-i", "x", "-n", "3", "-d", "0"], "distance must be a valid number larger than 0"),
+    (["-i", "x", "-n", "3", "-d", "-2"], "distance must be a valid number larger than 0"),
+    (["-i", "x", "-n", "3", "-m", "lev"], "invalid measure option"),
+    (["-i", "x", "-n", "3", "-N", "0"], "invalid info length"),
+    (["-i", "x", "-n", "3", "-I", "info.csv"], "repertiore path or repertoire count is missing in overlap mode"),
+    (["-i", "x", "-n", "3", "-N", "2"], "repertiore path or repertoire count is missing in overlap mode"),
+    (["-i", "/nonexistent/file", "-n", "3"], "input file reading failed"),
+])
+def test_cli_argument_errors(args, msg):
+    r = run_cli(args)
+    assert r.returncode == 1
+    assert r.stderr == f"Error: {msg}\n".encode()
+    assert r.stdout == b""
+
+
+def test_cli_invalid_residue():
+    path = os.path.join(GOLDEN, "sample_input3.txt")  # CAAA / AAAA / AAAX, header skipped
+    r = run_cli(["-i", path, "-n", "2"])
+    assert r.returncode == 1
+    assert r.stderr == b"Error: invalid char\nError: input parsing error\nError line: 3\n"
+
+
+def test_cli_length_mismatch(tmp_path):
+    p = write(tmp_path, "in.csv", "cdr3\nCAAA\nCADA\n\n \nCDKD\n")
+    r = run_cli(["-i", p, "-n", "4"])
+    assert r.returncode == 1
+    assert r.stderr == b"Error: input length doesn't match with the actual\n"
+    # more rows than -n overflow the sequence store first
+    r = run_cli(["-i", p, "-n", "2"])
+    assert r.stderr == b"Error: SeqArray exceeding max length\nError: input parsing error\nError line: 6\n"
+
+
+def test_cli_crlf_rejected(tmp_path):
+    p = write(tmp_path, "in.csv", "cdr3\r\nCAAA\r\n")
+    r = run_cli(["-i", p, "-n", "1"])
+    assert r.stderr == b"Error: invalid char\nError: input parsing error\nError line: 2\n"
+
+
+def test_cli_overlap_parse_errors(tmp_path):
+    seqs = write(tmp_path, "seqs.csv", "cdr3,count\nCAAA,3\nCADA,x\n")
+    info = write(tmp_path, "info.csv", "size\n1\n1\n")
+    r = run_cli(["-i", seqs, "-n", "2", "-I", info, "-N", "2"])
+    assert r.stderr == b"Error: input parsing error (invalid number)\nError line: 3\n"
+    seqs2 = write(tmp_path, "seqs2.csv", "cdr3,count\nCAAA,3\nCADA\n")
+    r = run_cli(["-i", seqs2, "-n", "2", "-I", info, "-N", "2"])
+    assert r.stderr == b"Error: input parsing error (comma expected)\nError line: 3\n"
+    bad_info = write(tmp_path, "bad.csv", "size\n1\n0\n")
+    r = run_cli(["-i", seqs, "-n", "2", "-I", bad_info, "-N", "2"])
+    assert r.stderr == b"Error: info parsing error (invalid number)\nError line: 3\n"
+    big_info = write(tmp_path, "big.csv", "size\n2\n1\n")
+    r = run_cli(["-i", seqs, "-n", "2", "-I", big_info, "-N", "2"])
+    assert r.stderr == b"Error: total repertoires' size does not match sequence count [1]\n"
+    short_info = write(tmp_path, "short.csv", "size\n1\n")
+    r = run_cli(["-i", seqs, "-n", "2", "-I", short_info, "-N", "2"])
+    assert r.stderr == b"Error: info length doesn't match with the actual\n"
+    small_info = write(tmp_path, "small.csv", "size\n1\n")
+    r = run_cli(["-i", seqs, "-n", "2", "-I", small_info, "-N", "1"])
+    assert r.stderr == b"Error: total repertoires' size does not match sequence count [2]\n"
+    r = run_cli(["-i", seqs, "-n", "2", "-I", "/nonexistent", "-N", "1"])
+    assert r.stderr == b"Error: info file reading failed\n"
+
+
+def test_cli_airr_parse_errors(tmp_path):
+    hdr = "a\tduplicate_count\tcdr3_aa\n"
+    bad_seq = write(tmp_path, "a1.tsv", hdr + "x\t1\tCAAA\ny\t2\tCAXA\n")
+    r = run_cli(["-a", "-i", bad_seq, "-n", "2"])
+    # data rows are numbered from 0 in AIRR mode (airr.cu:122, 130)
+    assert r.stderr == (b"Error: invalid char\nError: invalid seq\nError line: 1\n"
+                        b"Error: line parsing error\nError line: 1\n")
+    short = write(tmp_path, "a2.tsv", hdr + "x\t1\n")
+    r = run_cli(["-a", "-i", short, "-n", "1"])
+    assert r.stderr == b"Error: line parsing error\nError line: 0\n"
+    info = write(tmp_path, "info.csv", "size\n2\n")
+    bad_num = write(tmp_path, "a3.tsv", hdr + "x\t1\tCAAA\ny\t0\tCADA\n")
+    r = run_cli(["-a", "-i", bad_num, "-n", "2", "-I", info, "-N", "1"])
+    assert r.stderr == b"Error: invalid number\nError line: 1\nError: line parsing error\nError line: 1\n"

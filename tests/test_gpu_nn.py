@@ -1,0 +1,162 @@
+"""GPU parity of nearest-neighbour mode: the HIP path through the C ABI
+against the oracle (reference algorithm restated) and the brute-force
+definition, bit-exact on (i, j, distance) triples."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import xtn
+from conftest import GOLDEN, read_golden
+from helpers import AA20, clustered_seqs, random_seqs, triples
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine():
+    e = xtn.Engine(distance=1)
+    yield e
+    e.close()
+
+
+def run(engine, seqs=None, chars=None, offsets=None, d=1, measure=0, **kw):
+    engine.configure(distance=d, measure=measure, max_records_per_chunk=kw.get("chunk", 0),
+                     output_chunk_pairs=kw.get("out_chunk", 0))
+    engine.set_sequences(seqs, chars, offsets)
+    p, dist, ncb = engine.nn()
+    return triples(p, dist), ncb
+
+
+def test_dummy_fixture(engine):
+    seqs = [s for s in read_golden("dummy_input.txt").decode().split("\n")[1:] if s]
+    for d, measure, fx in [(1, 0, "expected_dummy_d1.txt"), (1, 1, "expected_dummy_d1.txt"),
+                           (2, 0, "expected_dummy_d2.txt")]:
+        got, _ = run(engine, seqs, d=d, measure=measure)
+        text = "".join(f"{a} {b} {c}\n" for a, b, c in got.tolist()).encode()
+        assert text == read_golden(fx)
+
+
+def test_stage4_vector(engine, golden):
+    g = golden["stream4"]
+    got, _ = run(engine, g["seqs"], d=g["distance"])
+    assert got.tolist() == g["expected"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_random_vs_brute(engine, seed, d):
+    rng = np.random.default_rng(1000 * seed + d)
+    alphabet = ["ACDE", AA20, "AC", AA20][seed % 4]
+    seqs = clustered_seqs(rng, 600, 1, 32, alphabet, edits=d + 1)
+    chars, offsets = xtn.encode(seqs)
+    for measure in (0, 1):
+        got, _ = run(engine, chars=chars, offsets=offsets, d=d, measure=measure)
+        want = oracle.nn_brute(chars, offsets, d, measure)
+        assert np.array_equal(got, want), (seed, d, measure)
+
+
+def test_large_groups_use_flattened_pairs(engine):
+    # a tiny alphabet makes long runs of equal keys (> 32 records)
+    rng = np.random.default_rng(5)
+    seqs = random_seqs(rng, 1500, 2, 6, alphabet="AC")
+    chars, offsets = xtn.encode(seqs)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=2)
+    assert np.array_equal(got, oracle.nn_brute(chars, offsets, 2, 0))
+
+
+@pytest.mark.parametrize("chunk", [1 << 16, 70000])
+def test_multi_chunk_equals_single(engine, chunk):
+    chars, offsets = xtn.synth_nn(3, 20000, planted=0.25, max_edits=2)
+    single, _ = run(engine, chars=chars, offsets=offsets, d=2)
+    multi, _ = run(engine, chars=chars, offsets=offsets, d=2, chunk=chunk)
+    assert engine.stats()["n_chunks"] > 1
+    assert np.array_equal(single, multi)
+    port, _ = oracle.nn_port(chars, offsets, 2, 0)
+    assert np.array_equal(single, port)
+
+
+def test_output_chunks_are_ascending(engine):
+    chars, offsets = xtn.synth_nn(4, 20000, planted=0.3)
+    full, ncb1 = run(engine, chars=chars, offsets=offsets, d=1)
+    parts, ncb = run(engine, chars=chars, offsets=offsets, d=1, out_chunk=1000)
+    assert ncb1 == 1 and ncb == (len(full) + 999) // 1000
+    assert np.array_equal(full, parts)
+
+
+def test_config1_10k(engine):
+    """10,000 synthetic CDR3, Levenshtein d=1 (the CPU-runnable config)"""
+    chars, offsets = xtn.synth_nn(1, 10000, planted=0.2, max_edits=1)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=1)
+    port, st = oracle.nn_port(chars, offsets, 1, 0)
+    assert np.array_equal(got, port)
+    assert np.array_equal(got, oracle.nn_brute(chars, offsets, 1, 0))
+    s = engine.stats()
+    assert s["n_variants"] == st["n_variants"]
+    assert s["n_output"] == len(port)
+
+
+@pytest.mark.parametrize("measure", [0, 1])
+def test_100k_d2_vs_port(engine, measure):
+    chars, offsets = xtn.synth_nn(2, 100000, planted=0.25, max_edits=2)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=2, measure=measure)
+    port, _ = oracle.nn_port(chars, offsets, 2, measure)
+    assert np.array_equal(got, port)
+
+
+def test_shuffle_invariance(engine):
+    """01_correctness.ipynb cell 18: permute the input, map indexes back"""
+    chars, offsets = xtn.synth_nn(9, 50000, planted=0.25, max_edits=2)
+    seqs = xtn.decode(chars, offsets)
+    base, _ = run(engine, chars=chars, offsets=offsets, d=2)
+    perm = np.random.default_rng(0).permutation(len(seqs))
+    got, _ = run(engine, [seqs[i] for i in perm], d=2)
+    i, j = perm[got[:, 0]], perm[got[:, 1]]
+    remapped = np.stack([np.minimum(i, j), np.maximum(i, j), got[:, 2]], axis=1)
+    remapped = remapped[np.lexsort((remapped[:, 1], remapped[:, 0]))]
+    assert np.array_equal(remapped, base)
+
+
+def test_edge_cases(engine):
+    # empty input, one sequence, duplicates, single residues, long strings
+    assert run(engine, [], d=1)[0].shape == (0, 3)
+    assert run(engine, ["CASS"], d=1)[0].shape == (0, 3)
+    got, _ = run(engine, ["CASS", "CASS", "CASS"], d=1)
+    assert got.tolist() == [[0, 1, 0], [0, 2, 0], [1, 2, 0]]
+    got, _ = run(engine, ["A", "C", "AC", "D"], d=1)
+    assert np.array_equal(got, oracle.nn_brute(*xtn.encode(["A", "C", "AC", "D"]), 1, 0))
+    rng = np.random.default_rng(11)
+    longs = clustered_seqs(rng, 300, 17, 120, "ACDEF", edits=2)
+    chars, offsets = xtn.encode(longs)
+    for d in (1, 2, 4):
+        got, _ = run(engine, chars=chars, offsets=offsets, d=d)
+        assert np.array_equal(got, oracle.nn_brute(chars, offsets, d, 0)), d
+    # distances above the register band templates (generic path)
+    got, _ = run(engine, chars=chars[:offsets[80]], offsets=offsets[:81], d=10)
+    assert np.array_equal(got, oracle.nn_brute(chars[:offsets[80]], offsets[:81], 10, 0))
+
+
+def test_invalid_residue_rejected(engine):
+    with pytest.raises(xtn.XtnError) as e:
+        engine.set_sequences(["CASS", "CAXS"])
+    assert e.value.status == 1
+
+
+def test_cli_end_to_end(tmp_path):
+    env = dict(os.environ, LC_ALL="C")
+    for d, measure, fx in [("1", "leven", "expected_dummy_d1.txt"), ("1", "hamming", "expected_dummy_d1.txt"),
+                           ("2", "leven", "expected_dummy_d2.txt")]:
+        out = tmp_path / f"o{d}{measure}.txt"
+        r = subprocess.run([xtn.CLI_PATH, "-i", os.path.join(GOLDEN, "dummy_input.txt"), "-n", "5",
+                            "-d", d, "-m", measure, "-o", str(out)], capture_output=True, env=env, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout == b"" and r.stderr == b""
+        assert out.read_bytes() == read_golden(fx)
+    # no -o: nothing written, exit 0; -V ends with the total line
+    r = subprocess.run([xtn.CLI_PATH, "-i", os.path.join(GOLDEN, "dummy_input.txt"), "-n", "5", "-V"],
+                       capture_output=True, env=env, timeout=120)
+    assert r.returncode == 0
+    assert r.stdout.startswith(b"XTNArgs{\n\tdistance: 1\n")
+    assert r.stdout.endswith(b"total output length: 5\n")

@@ -1,0 +1,599 @@
+// xtn_kernels.hpp -- the XTNeighbor hot-path kernels for gfx950.
+//
+// Data layout in HBM (one key-space chunk):
+//   pk[2*i], pk[2*i+1]   18-residue prefix of sequence i packed as a 90-bit
+//                        integer (residue 0 most significant, 5 bits each,
+//                        code = c - '@'), prefix length in bits 56..63 of the
+//                        high word.
+//   voff[i]              first deletion record of sequence i (exclusive scan
+//                        of the per-sequence variant counts), voff[n] = V.
+//   rec[r]               one deletion record: 64-bit word, the high 64-idb
+//                        bits are a hash of the variant's packed prefix, the
+//                        low idb bits the sequence id.
+//   pairs[q]             candidate pair (i << idb) | j, i < j.
+//
+// A hashed key only widens the candidate set (two variants that collide
+// produce extra candidates that verification removes); every pair of strings
+// within distance d still shares an exact variant of their 18-prefixes, so
+// the verified output is unchanged. The reference keys the same variants
+// exactly as a 12-byte Int3 (xtn.h:19-25, kernel.cu:594-610).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xtn_prims.hpp"
+
+namespace xtn {
+
+constexpr int MAX_PREFIX = 18;      // xtn.h:8 MAX_COMPRESSED_LENGTH
+constexpr int HAMMING_SENTINEL = 77;  // kernel.cu:292
+constexpr int SMALL_RUN_MAX = 32;   // runs longer than this use the flattened pair kernel
+
+typedef unsigned __int128 u128;
+
+struct RunCtrs {
+	unsigned long long n_small;      // runs of 2..SMALL_RUN_MAX records
+	unsigned long long n_big;        // longer runs
+	unsigned long long pairs_small;  // sum len*(len-1)/2 over small runs
+	unsigned long long pairs_big;    // same over big runs
+	unsigned long long n_cand;       // candidate pairs written (append cursor)
+	unsigned long long n_unique;     // unique candidate pairs verified
+	unsigned long long append;       // partition append cursor
+	unsigned long long pad;
+};
+
+struct Run {
+	uint32_t start, len;
+};
+
+// ------------------------------------------------------------ packing --
+
+__device__ __forceinline__ uint64_t variant_count(int L, int d) {
+	const int m = d < L ? d : L;
+	uint64_t c = 1, total = 1;
+	for (int j = 1; j <= m; ++j) {
+		c = c * (uint64_t)(L - j + 1) / (uint64_t)j;
+		total += c;
+	}
+	return total;
+}
+
+// one thread per sequence: pack the 18-prefix, count its deletion variants
+// (kernel.cu:33-57 counts the same sum of binomials)
+__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
+                                              const uint32_t* __restrict__ offs,
+                                              const uint32_t* __restrict__ seq_map, int32_t n,
+                                              int distance, uint64_t* __restrict__ pk,
+                                              uint64_t* __restrict__ vcount) {
+	const int32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n)
+		return;
+	const uint32_t o = seq_map ? seq_map[i] : (uint32_t)i;
+	const uint32_t s = offs[o];
+	const uint32_t len = offs[o + 1] - s;
+	const int L = len < (uint32_t)MAX_PREFIX ? (int)len : MAX_PREFIX;
+	uint64_t lo = 0, hi = 0;
+	for (int k = 0; k < L; ++k) {
+		const uint64_t code = (uint64_t)(chars[s + k] - '@') & 0x1Full;
+		const int sh = 5 * (MAX_PREFIX - 1 - k);
+		if (sh >= 64)
+			hi |= code << (sh - 64);
+		else {
+			lo |= code << sh;
+			if (sh > 59)
+				hi |= code >> (64 - sh);
+		}
+	}
+	pk[2 * (int64_t)i] = lo;
+	pk[2 * (int64_t)i + 1] = hi | ((uint64_t)L << 56);
+	vcount[i] = variant_count(L, distance);
+}
+
+// ---------------------------------------------------------- generation --
+
+__device__ __forceinline__ u128 del_slot(u128 p, int pos) {
+	// slot pos occupies bits [5*(17-pos), 5*(17-pos)+5)
+	const int low_bits = 5 * (MAX_PREFIX - 1 - pos);
+	const u128 one = 1;
+	const u128 low_mask = (one << low_bits) - 1;
+	const u128 high_mask = ~((one << (low_bits + 5)) - 1);
+	return (p & high_mask) | ((p & low_mask) << 5);
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+	x ^= x >> 32;
+	x *= 0xD6E8FEB86659FD93ull;
+	x ^= x >> 32;
+	x *= 0xD6E8FEB86659FD93ull;
+	x ^= x >> 32;
+	return x;
+}
+
+__device__ __forceinline__ uint64_t hash_variant(u128 v) {
+	const uint64_t lo = (uint64_t)v;
+	const uint64_t hi = (uint64_t)(v >> 64);
+	return mix64(lo ^ (hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull);
+}
+
+// the k-th deletion variant of a prefix of length L with at most m deletions.
+// Ranks enumerate deletion sets by size (0, 1, 2, ...), each size in colex
+// order. The order is internal: records are sorted by key afterwards.
+__device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
+                                               const uint32_t* __restrict__ binom /*19x19*/) {
+	if (k == 0)
+		return p;
+	k -= 1;
+	if (k < (uint32_t)L)
+		return del_slot(p, (int)k);
+	k -= (uint32_t)L;
+	if (m >= 2) {
+		const uint32_t c2 = binom[L * 19 + 2];
+		if (k < c2) {
+			// largest b with b(b-1)/2 <= k
+			uint32_t b = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)k)) * 0.5f);
+			while (b * (b - 1) / 2 > k)
+				--b;
+			while ((b + 1) * b / 2 <= k)
+				++b;
+			const uint32_t a = k - b * (b - 1) / 2;
+			return del_slot(del_slot(p, (int)b), (int)a);
+		}
+		k -= c2;
+	}
+	for (int j = 3; j <= m; ++j) {
+		const uint32_t cj = binom[L * 19 + j];
+		if (k < cj) {
+			u128 v = p;
+			uint32_t q = k;
+			for (int t = j; t >= 1; --t) {
+				int c = t - 1;
+				while (c + 1 < L && binom[(c + 1) * 19 + t] <= q)
+					++c;
+				q -= binom[c * 19 + t];
+				v = del_slot(v, c);  // positions come out in decreasing order
+			}
+			return v;
+		}
+		k -= cj;
+	}
+	return p;  // unreachable for k < variant_count
+}
+
+constexpr int GEN_NT = 256;
+constexpr int GEN_IT = 8;
+constexpr int GEN_TILE = GEN_NT * GEN_IT;
+
+enum GenMode { GEN_DIRECT = 0, GEN_HIST = 1, GEN_APPEND = 2 };
+
+struct GenArgs {
+	const uint64_t* pk;
+	const uint64_t* voff;  // n+1 entries
+	int32_t n;
+	int32_t distance;
+	int32_t idb;
+	int32_t part_bits;  // partition = top part_bits of the hash
+	uint32_t part_id;
+	int32_t pad;
+	int64_t V;
+	uint64_t* out;
+	int64_t cap;
+	unsigned long long* part_hist;  // GEN_HIST: 2^part_bits counters
+	unsigned long long* append;     // GEN_APPEND cursor
+};
+
+__device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n, uint64_t x) {
+	int64_t lo = 0, hi = n;
+	while (lo < hi) {
+		int64_t mid = (lo + hi) >> 1;
+		if (a[mid] <= x)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// Flattened generation: each workgroup owns GEN_TILE consecutive record
+// indexes, so stores are contiguous whatever the sequence lengths
+// (generate_combination.cu:212-229 instead runs one thread per sequence and
+// writes a strided run per lane).
+template <int MODE>
+__global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
+	__shared__ uint64_t s_voff[GEN_TILE + 2];
+	__shared__ uint32_t s_binom[19 * 19];
+	__shared__ int64_t s_lo;
+	__shared__ int s_cnt;
+	__shared__ unsigned long long s_hist[MODE == GEN_HIST ? 4096 : 1];
+
+	const int t = threadIdx.x;
+	const int64_t r0 = (int64_t)blockIdx.x * GEN_TILE;
+	if (r0 >= a.V)
+		return;
+	const int64_t r1 = r0 + GEN_TILE < a.V ? r0 + GEN_TILE : a.V;
+
+	for (int i = t; i < 19 * 19; i += GEN_NT) {
+		const int nn = i / 19, kk = i % 19;
+		uint32_t c = 0;
+		if (kk <= nn) {
+			c = 1;
+			for (int j = 1; j <= kk; ++j)
+				c = c * (uint32_t)(nn - j + 1) / (uint32_t)j;
+		}
+		s_binom[i] = c;
+	}
+	if (MODE == GEN_HIST) {
+		const int nb = 1 << a.part_bits;
+		for (int i = t; i < nb; i += GEN_NT)
+			s_hist[i] = 0;
+	}
+	if (t == 0) {
+		const int64_t lo = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)r0) - 1;
+		const int64_t hi = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)(r1 - 1)) - 1;
+		s_lo = lo;
+		s_cnt = (int)(hi - lo + 1);
+	}
+	__syncthreads();
+	const int64_t slo = s_lo;
+	const int cnt = s_cnt;  // sequences touching this tile, <= GEN_TILE
+	for (int i = t; i <= cnt; i += GEN_NT)
+		s_voff[i] = a.voff[slo + i];
+	__syncthreads();
+
+	const uint64_t idmask = (1ull << a.idb) - 1ull;
+	int idx = 0;
+	for (int it = 0; it < GEN_IT; ++it) {
+		const int64_t r = r0 + (int64_t)it * GEN_NT + t;
+		const bool valid = r < r1;
+		uint64_t rec = 0;
+		if (valid) {
+			// records of a lane only move forward: advance the sequence cursor
+			if (it == 0) {
+				int lo = 0, hi = cnt;  // largest idx with s_voff[idx] <= r
+				while (lo < hi) {
+					int mid = (lo + hi + 1) >> 1;
+					if (s_voff[mid] <= (uint64_t)r)
+						lo = mid;
+					else
+						hi = mid - 1;
+				}
+				idx = lo;
+			} else {
+				while (idx + 1 < cnt && s_voff[idx + 1] <= (uint64_t)r)
+					++idx;
+			}
+			const int64_t sid = slo + idx;
+			const uint32_t k = (uint32_t)((uint64_t)r - s_voff[idx]);
+			const uint64_t lo64 = a.pk[2 * sid];
+			const uint64_t hiw = a.pk[2 * sid + 1];
+			const int L = (int)(hiw >> 56);
+			const int m = a.distance < L ? a.distance : L;
+			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
+			const u128 v = unrank_variant(p, L, m, k, s_binom);
+			rec = (hash_variant(v) & ~idmask) | ((uint64_t)sid & idmask);
+		}
+		if (MODE == GEN_DIRECT) {
+			if (valid)
+				a.out[r] = rec;
+		} else if (MODE == GEN_HIST) {
+			if (valid)
+				atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1ull);
+		} else {
+			const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
+			const unsigned long long slot = wave_append(a.append, mine ? 1u : 0u);
+			if (mine && (int64_t)slot < a.cap)
+				a.out[slot] = rec;
+		}
+	}
+	if (MODE == GEN_HIST) {
+		__syncthreads();
+		const int nb = 1 << a.part_bits;
+		for (int i = t; i < nb; i += GEN_NT)
+			if (s_hist[i])
+				atomicAdd(&a.part_hist[i], s_hist[i]);
+	}
+}
+
+// --------------------------------------------------------------- runs --
+
+// one thread per sorted record: a run is a maximal segment of equal
+// (rec >> rs); runs of >= 2 records are listed (xtn_inner.cu:34-46 does this
+// with DeviceRunLengthEncode over all keys)
+__global__ __launch_bounds__(256) void k_runs(const uint64_t* __restrict__ rec, int64_t n, int rs,
+                                              Run* __restrict__ small, int64_t small_cap,
+                                              Run* __restrict__ big, int64_t big_cap,
+                                              RunCtrs* __restrict__ ctr) {
+	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	bool is_small = false, is_big = false;
+	uint32_t len = 0;
+	if (i < n) {
+		const uint64_t cur = rec[i] >> rs;
+		const bool head = i == 0 || (rec[i - 1] >> rs) != cur;
+		if (head && i + 1 < n && (rec[i + 1] >> rs) == cur) {
+			int64_t j = i + 2;
+			while (j < n && (rec[j] >> rs) == cur)
+				++j;
+			len = (uint32_t)(j - i);
+			is_small = len <= (uint32_t)SMALL_RUN_MAX;
+			is_big = !is_small;
+		}
+	}
+	const unsigned long long ps = wave_append(&ctr->n_small, is_small ? 1u : 0u);
+	if (is_small && (int64_t)ps < small_cap)
+		small[ps] = Run{(uint32_t)i, len};
+	const unsigned long long pb = wave_append(&ctr->n_big, is_big ? 1u : 0u);
+	if (is_big && (int64_t)pb < big_cap)
+		big[pb] = Run{(uint32_t)i, len};
+	const unsigned long long np = (unsigned long long)len * (len > 0 ? len - 1 : 0) / 2;
+	// pair bounds: small runs stay far below 2^32 per lane, big runs may not
+	unsigned long long sp = is_small ? np : 0, bp = is_big ? np : 0;
+	for (int o = WAVE / 2; o > 0; o >>= 1) {
+		sp += __shfl_down(sp, o, WAVE);
+		bp += __shfl_down(bp, o, WAVE);
+	}
+	if (lane_id() == 0) {
+		if (sp)
+			atomicAdd(&ctr->pairs_small, sp);
+		if (bp)
+			atomicAdd(&ctr->pairs_big, bp);
+	}
+}
+
+__device__ __forceinline__ bool pair_of(uint64_t ka, uint64_t kb, int idb, uint64_t* out) {
+	if ((ka >> idb) != (kb >> idb))
+		return false;  // same sorted bits, different variant hash
+	const uint64_t mask = (1ull << idb) - 1ull;
+	const uint64_t ia = ka & mask, ib = kb & mask;
+	if (ia == ib)
+		return false;  // a sequence paired with itself (kernel.cu:377-380 drops these later)
+	*out = ia < ib ? ((ia << idb) | ib) : ((ib << idb) | ia);
+	return true;
+}
+
+// every unordered record pair of a small run (kernel.cu:157-191 does this for
+// every group, one thread per group)
+__global__ __launch_bounds__(256) void k_pairs_small(const uint64_t* __restrict__ rec,
+                                                     const Run* __restrict__ runs, int64_t nruns,
+                                                     int idb, uint64_t* __restrict__ pairs,
+                                                     int64_t cap, RunCtrs* __restrict__ ctr) {
+	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	Run run{0, 0};
+	if (t < nruns)
+		run = runs[t];
+	uint32_t count = 0;
+	for (uint32_t a = 0; a < run.len; ++a) {
+		const uint64_t ka = rec[run.start + a];
+		for (uint32_t b = a + 1; b < run.len; ++b) {
+			uint64_t p;
+			count += pair_of(ka, rec[run.start + b], idb, &p) ? 1u : 0u;
+		}
+	}
+	unsigned long long slot = wave_append(&ctr->n_cand, count);
+	for (uint32_t a = 0; a < run.len; ++a) {
+		const uint64_t ka = rec[run.start + a];
+		for (uint32_t b = a + 1; b < run.len; ++b) {
+			uint64_t p;
+			if (pair_of(ka, rec[run.start + b], idb, &p)) {
+				if ((int64_t)slot < cap)
+					pairs[slot] = p;
+				++slot;
+			}
+		}
+	}
+}
+
+__global__ __launch_bounds__(256) void k_big_counts(const Run* __restrict__ runs, int64_t nruns,
+                                                    uint64_t* __restrict__ counts) {
+	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (t < nruns) {
+		const uint64_t l = runs[t].len;
+		counts[t] = l * (l - 1) / 2;
+	}
+}
+
+// bounded fan-out: the pairs of all long runs form one flat index space that
+// the whole grid strides over, so a single huge group is spread over every CU
+__global__ __launch_bounds__(256) void k_pairs_big(const uint64_t* __restrict__ rec,
+                                                   const Run* __restrict__ runs,
+                                                   const uint64_t* __restrict__ pre /* nruns+1 */,
+                                                   int64_t nruns, int idb,
+                                                   uint64_t* __restrict__ pairs, int64_t cap,
+                                                   RunCtrs* __restrict__ ctr) {
+	const uint64_t total = pre[nruns];
+	const uint64_t stride = (uint64_t)gridDim.x * 256;
+	for (uint64_t base = (uint64_t)blockIdx.x * 256; base < total; base += stride) {
+		const uint64_t q = base + threadIdx.x;
+		bool ok = false;
+		uint64_t p = 0;
+		if (q < total) {
+			int64_t lo = 0, hi = nruns - 1;  // run with pre[run] <= q < pre[run+1]
+			while (lo < hi) {
+				int64_t mid = (lo + hi + 1) >> 1;
+				if (pre[mid] <= q)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			const uint64_t u = q - pre[lo];
+			uint64_t b = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)u)) * 0.5);
+			while (b * (b - 1) / 2 > u)
+				--b;
+			while ((b + 1) * b / 2 <= u)
+				++b;
+			const uint64_t a = u - b * (b - 1) / 2;
+			const uint32_t st = runs[lo].start;
+			ok = pair_of(rec[st + a], rec[st + b], idb, &p);
+		}
+		const unsigned long long slot = wave_append(&ctr->n_cand, ok ? 1u : 0u);
+		if (ok && (int64_t)slot < cap)
+			pairs[slot] = p;
+	}
+}
+
+// ------------------------------------------------------- verification --
+
+// Levenshtein restricted to the diagonal band |x - y| <= D: exact when the
+// distance is <= D, otherwise some value > D (kernel.cu:235-280 computes the
+// full Wagner-Fischer table). The band lives in registers, one pair per lane.
+template <int D>
+__device__ __forceinline__ int lev_band(const uint8_t* __restrict__ a, int la,
+                                        const uint8_t* __restrict__ b, int lb) {
+	constexpr int W = 2 * D + 1;
+	constexpr int INF = D + 1;
+	if (la - lb > D || lb - la > D)
+		return INF;
+	int row[W];
+#pragma unroll
+	for (int k = 0; k < W; ++k) {
+		const int x = k - D;  // row y = 0: cell (0, x)
+		row[k] = (x >= 0 && x <= lb) ? (x < INF ? x : INF) : INF;
+	}
+	for (int y = 1; y <= la; ++y) {
+		const uint8_t ca = a[y - 1];
+		int left = INF;
+#pragma unroll
+		for (int k = 0; k < W; ++k) {
+			const int x = y - D + k;
+			int val = INF;
+			if (x >= 0 && x <= lb) {
+				if (x == 0) {
+					val = y < INF ? y : INF;
+				} else {
+					const int diag = row[k] + (ca != b[x - 1] ? 1 : 0);
+					const int up = (k + 1 < W ? row[k + 1] : INF) + 1;
+					val = diag < up ? diag : up;
+					val = left + 1 < val ? left + 1 : val;
+					val = val < INF ? val : INF;
+				}
+			}
+			row[k] = val;  // row[k] of the previous line is no longer needed
+			left = val;
+		}
+	}
+	return row[lb - la + D];
+}
+
+// the same band with a run-time half width D <= 127 (distances beyond the
+// register templates); the band is 2D+1 cells of private memory
+__device__ int lev_band_rt(const uint8_t* __restrict__ a, int la, const uint8_t* __restrict__ b,
+                           int lb, int D) {
+	const int W = 2 * D + 1;
+	const int INF = D + 1;
+	if (la - lb > D || lb - la > D)
+		return INF;
+	int row[255];
+	for (int k = 0; k < W; ++k) {
+		const int x = k - D;
+		row[k] = (x >= 0 && x <= lb) ? (x < INF ? x : INF) : INF;
+	}
+	for (int y = 1; y <= la; ++y) {
+		const uint8_t ca = a[y - 1];
+		int left = INF;
+		for (int k = 0; k < W; ++k) {
+			const int x = y - D + k;
+			int val = INF;
+			if (x >= 0 && x <= lb) {
+				if (x == 0) {
+					val = y < INF ? y : INF;
+				} else {
+					const int diag = row[k] + (ca != b[x - 1] ? 1 : 0);
+					const int up = (k + 1 < W ? row[k + 1] : INF) + 1;
+					val = diag < up ? diag : up;
+					val = left + 1 < val ? left + 1 : val;
+					val = val < INF ? val : INF;
+				}
+			}
+			row[k] = val;
+			left = val;
+		}
+	}
+	return row[lb - la + D];
+}
+
+__device__ __forceinline__ int hamming(const uint8_t* __restrict__ a, int la,
+                                       const uint8_t* __restrict__ b, int lb) {
+	if (la != lb)
+		return HAMMING_SENTINEL;  // kernel.cu:291-292
+	int d = 0;
+	for (int i = 0; i < la; ++i)
+		d += a[i] != b[i];
+	return d;
+}
+
+__device__ __forceinline__ int seq_distance(const uint8_t* a, int la, const uint8_t* b, int lb,
+                                            int distance, int measure) {
+	if (measure == 1)
+		return hamming(a, la, b, lb);
+	switch (distance) {
+	case 1: return lev_band<1>(a, la, b, lb);
+	case 2: return lev_band<2>(a, la, b, lb);
+	case 3: return lev_band<3>(a, la, b, lb);
+	case 4: return lev_band<4>(a, la, b, lb);
+	case 5: return lev_band<5>(a, la, b, lb);
+	case 6: return lev_band<6>(a, la, b, lb);
+	case 7: return lev_band<7>(a, la, b, lb);
+	case 8: return lev_band<8>(a, la, b, lb);
+	default: return lev_band_rt(a, la, b, lb, distance);
+	}
+}
+
+// one lane per sorted candidate pair: duplicates of the previous pair are
+// skipped (DeviceSelect::Unique, xtn_inner.cu:138-142), the rest verified on
+// the full strings and flagged when distance <= d (kernel.cu:359-392)
+__global__ __launch_bounds__(256) void k_verify(const uint64_t* __restrict__ pairs, int64_t n,
+                                                int idb, const uint8_t* __restrict__ chars,
+                                                const uint32_t* __restrict__ offs,
+                                                const uint32_t* __restrict__ seq_map, int distance,
+                                                int measure, uint32_t* __restrict__ flags,
+                                                int8_t* __restrict__ dist,
+                                                RunCtrs* __restrict__ ctr) {
+	const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	bool uniq = false;
+	uint32_t keep = 0;
+	int8_t dv = 0;
+	if (q < n) {
+		const uint64_t p = pairs[q];
+		uniq = q == 0 || pairs[q - 1] != p;
+		if (uniq) {
+			const uint64_t mask = (1ull << idb) - 1ull;
+			const uint32_t i = (uint32_t)(p >> idb), j = (uint32_t)(p & mask);
+			const uint32_t oi = seq_map ? seq_map[i] : i;
+			const uint32_t oj = seq_map ? seq_map[j] : j;
+			const uint32_t si = offs[oi], sj = offs[oj];
+			const int li = (int)(offs[oi + 1] - si), lj = (int)(offs[oj + 1] - sj);
+			const int d = seq_distance(chars + si, li, chars + sj, lj, distance, measure);
+			keep = d <= distance ? 1u : 0u;
+			dv = (int8_t)d;
+		}
+	}
+	unsigned long long c = (unsigned long long)__popcll(__ballot(uniq));
+	if (lane_id() == 0 && c)
+		atomicAdd(&ctr->n_unique, c);
+	if (q < n) {
+		flags[q] = keep;
+		dist[q] = dv;
+	}
+}
+
+__global__ __launch_bounds__(256) void k_emit_nn(const uint64_t* __restrict__ pairs, int64_t n,
+                                                 int idb, const uint32_t* __restrict__ flags,
+                                                 const uint32_t* __restrict__ pos,
+                                                 const int8_t* __restrict__ dist,
+                                                 int2* __restrict__ out_pairs,
+                                                 int8_t* __restrict__ out_dist) {
+	const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (q >= n || !flags[q])
+		return;
+	const uint64_t p = pairs[q];
+	const uint32_t o = pos[q];
+	out_pairs[o] = make_int2((int)(p >> idb), (int)(p & ((1ull << idb) - 1ull)));
+	out_dist[o] = dist[q];
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
+	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+}  // namespace xtn

@@ -1,0 +1,841 @@
+// xtn_lib.hip -- host side of libxtn.so: handle, chunk scheduler, pipeline
+// orchestration and the C ABI declared in include/xtn.h.
+//
+// Pipeline for one run (NN mode; overlap mode runs it on the unique strings):
+//   k_pack + scan          sequences -> packed prefixes, variant offsets (V)
+//   per key-space chunk:   k_gen (records) -> radix sort (run key bits)
+//                          -> k_runs -> k_pairs_small / k_pairs_big
+//   radix sort of pairs -> k_verify (unique + distance) -> scan -> k_emit_nn
+// The reference runs the same stages as stream 1..4 of xtn_perform
+// (xtn.cu:267-509) with every intermediate staged through pinned host memory;
+// here everything between input upload and output download stays in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/xtn.h"
+#include "xtn_kernels.hpp"
+#include "xtn_overlap.hpp"
+
+using namespace xtn;
+
+namespace {
+
+struct DBuf {
+	void* p = nullptr;
+	size_t cap = 0;
+	template <typename T>
+	T* as() const {
+		return static_cast<T*>(p);
+	}
+};
+
+struct ProfAcc {
+	int64_t launches = 0;
+	double ms = 0.0;
+	double bytes = 0.0;
+};
+
+struct PendingEvent {
+	std::string name;
+	hipEvent_t start, stop;
+	double bytes;
+};
+
+}  // namespace
+
+struct xtn_handle {
+	xtn_config cfg;
+	hipStream_t stream = nullptr;
+	hipStream_t copy_stream = nullptr;
+	std::string err;
+
+	// device-resident input (xtn_set_sequences)
+	DBuf chars, offs;
+	int32_t n = 0;
+	int64_t n_chars = 0;
+
+	// workspaces (grow-only)
+	DBuf pk, voff, recs, recs_alt, hist, scan_tmp, runs_small, runs_big, big_pre, pairs, pairs_alt,
+	    flags, pos, dist, out_pairs, out_dist, ctrs, part_hist, scalar;
+	// overlap-mode workspaces
+	DBuf ov_keys, ov_keys_alt, ov_vals, ov_vals_alt, ov_info, ov_leader, ov_ustart, ov_umap,
+	    ov_cnt, ov_matrix, ov_present;
+
+	int64_t out_len = 0;
+	xtn_stats stats;
+
+	std::vector<PendingEvent> pending;
+	std::vector<hipEvent_t> event_pool;
+	std::map<std::string, ProfAcc> prof;
+};
+
+namespace {
+
+int fail(xtn_handle* h, int code, const char* fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	if (h)
+		h->err = buf;
+	return code;
+}
+
+#define HIP_TRY(h, expr)                                                                      \
+	do {                                                                                      \
+		hipError_t _e = (expr);                                                               \
+		if (_e != hipSuccess)                                                                 \
+			return fail((h), XTN_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+			            __FILE__, __LINE__);                                                  \
+	} while (0)
+
+#define HIP_LAUNCH_CHECK(h)                                                                   \
+	do {                                                                                      \
+		hipError_t _e = hipGetLastError();                                                    \
+		if (_e != hipSuccess)                                                                 \
+			return fail((h), XTN_ERR_HIP, "kernel launch failed: %s (%s:%d)",                 \
+			            hipGetErrorString(_e), __FILE__, __LINE__);                            \
+	} while (0)
+
+#define TRY(expr)                 \
+	do {                          \
+		int _rc = (expr);         \
+		if (_rc != XTN_OK)        \
+			return _rc;           \
+	} while (0)
+
+int ensure(xtn_handle* h, DBuf& b, size_t bytes) {
+	if (bytes == 0)
+		bytes = 256;
+	if (bytes <= b.cap)
+		return XTN_OK;
+	if (b.p) {
+		HIP_TRY(h, hipStreamSynchronize(h->stream));
+		HIP_TRY(h, hipFree(b.p));
+		b.p = nullptr;
+		b.cap = 0;
+	}
+	size_t want = bytes + bytes / 8;
+	want = (want + 4095) & ~(size_t)4095;
+	hipError_t e = hipMalloc(&b.p, want);
+	if (e != hipSuccess) {
+		(void)hipGetLastError();
+		want = (bytes + 4095) & ~(size_t)4095;
+		e = hipMalloc(&b.p, want);
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			b.p = nullptr;
+			return fail(h, XTN_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
+		}
+	}
+	b.cap = want;
+	return XTN_OK;
+}
+
+void release(DBuf& b) {
+	if (b.p)
+		(void)hipFree(b.p);
+	b.p = nullptr;
+	b.cap = 0;
+}
+
+// grow a buffer keeping its first `keep` bytes
+int grow_keep(xtn_handle* h, DBuf& b, size_t bytes, size_t keep) {
+	if (bytes <= b.cap)
+		return XTN_OK;
+	DBuf nb;
+	TRY(ensure(h, nb, bytes));
+	if (keep)
+		HIP_TRY(h, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, h->stream));
+	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	release(b);
+	b = nb;
+	return XTN_OK;
+}
+
+inline unsigned grid_for(int64_t n, int per_block) {
+	int64_t g = (n + per_block - 1) / per_block;
+	return (unsigned)(g > 0 ? g : 1);
+}
+
+// ------------------------------------------------------------ profiling --
+
+hipEvent_t take_event(xtn_handle* h) {
+	if (!h->event_pool.empty()) {
+		hipEvent_t e = h->event_pool.back();
+		h->event_pool.pop_back();
+		return e;
+	}
+	hipEvent_t e = nullptr;
+	if (hipEventCreate(&e) != hipSuccess)
+		return nullptr;
+	return e;
+}
+
+struct ProfScope {
+	xtn_handle* h;
+	PendingEvent pe;
+	bool on;
+	ProfScope(xtn_handle* hh, const char* name, double bytes) : h(hh), on(hh->cfg.profile != 0) {
+		if (!on)
+			return;
+		pe.name = name;
+		pe.bytes = bytes;
+		pe.start = take_event(h);
+		pe.stop = take_event(h);
+		if (!pe.start || !pe.stop) {
+			on = false;
+			return;
+		}
+		(void)hipEventRecord(pe.start, h->stream);
+	}
+	~ProfScope() {
+		if (!on)
+			return;
+		(void)hipEventRecord(pe.stop, h->stream);
+		h->pending.push_back(pe);
+	}
+};
+
+void collect_profile(xtn_handle* h) {
+	for (auto& pe : h->pending) {
+		float ms = 0.f;
+		if (hipEventSynchronize(pe.stop) == hipSuccess && hipEventElapsedTime(&ms, pe.start, pe.stop) == hipSuccess) {
+			ProfAcc& acc = h->prof[pe.name];
+			acc.launches += 1;
+			acc.ms += ms;
+			acc.bytes += pe.bytes;
+		}
+		h->event_pool.push_back(pe.start);
+		h->event_pool.push_back(pe.stop);
+	}
+	h->pending.clear();
+}
+
+// ------------------------------------------------------------ primitives --
+
+template <typename T>
+int scan_excl(xtn_handle* h, const T* in, T* out, int64_t n, T* d_total) {
+	hipStream_t s = h->stream;
+	if (n <= 0) {
+		if (d_total)
+			HIP_TRY(h, hipMemsetAsync(d_total, 0, sizeof(T), s));
+		return XTN_OK;
+	}
+	const int64_t nt = (n + SC_TILE - 1) / SC_TILE;
+	TRY(ensure(h, h->scan_tmp, sizeof(T) * (size_t)nt));
+	T* tmp = h->scan_tmp.as<T>();
+	k_scan_sums<T><<<grid_for(nt, 1), SC_NT, 0, s>>>(in, n, tmp);
+	HIP_LAUNCH_CHECK(h);
+	k_scan_single<T><<<1, SC_NT, 0, s>>>(tmp, nt, d_total);
+	HIP_LAUNCH_CHECK(h);
+	k_scan_apply<T><<<grid_for(nt, 1), SC_NT, 0, s>>>(in, out, n, tmp);
+	HIP_LAUNCH_CHECK(h);
+	return XTN_OK;
+}
+
+// Sorts n keys (and values when vals != nullptr) by bits [b0, b1), ping-
+// ponging between the buffer pairs; on return `keys`/`vals` name the buffers
+// holding the result.
+int radix_sort(xtn_handle* h, DBuf& keys, DBuf& keys_alt, DBuf* vals, DBuf* vals_alt, int64_t n,
+               int b0, int b1, const char* prof_name) {
+	if (n <= 1 || b0 >= b1)
+		return XTN_OK;
+	if (n > (int64_t)INT32_MAX)
+		return fail(h, XTN_ERR_CAPACITY, "radix sort of %lld keys exceeds 2^31", (long long)n);
+	hipStream_t s = h->stream;
+	const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+	TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * ntiles)));
+	const bool kv = vals != nullptr;
+	for (int shift = b0; shift < b1; shift += 8) {
+		const double key_bytes = (double)n * (8.0 + (kv ? 4.0 : 0.0));
+		{
+			std::string nm = std::string(prof_name) + "_hist";
+			ProfScope ps(h, nm.c_str(), key_bytes);
+			k_radix_hist<<<(unsigned)ntiles, RS_NT, 0, s>>>(keys.as<uint64_t>(), n, shift,
+			                                                 h->hist.as<uint32_t>(), ntiles);
+			HIP_LAUNCH_CHECK(h);
+		}
+		TRY(scan_excl<uint32_t>(h, h->hist.as<uint32_t>(), h->hist.as<uint32_t>(), RS_BINS * ntiles,
+		                        nullptr));
+		{
+			std::string nm = std::string(prof_name) + "_scatter";
+			ProfScope ps(h, nm.c_str(), 2.0 * key_bytes);
+			if (kv)
+				k_radix_scatter<true><<<(unsigned)ntiles, RS_NT, 0, s>>>(
+				    keys.as<uint64_t>(), vals->as<uint32_t>(), keys_alt.as<uint64_t>(),
+				    vals_alt->as<uint32_t>(), n, shift, h->hist.as<uint32_t>(), ntiles);
+			else
+				k_radix_scatter<false><<<(unsigned)ntiles, RS_NT, 0, s>>>(
+				    keys.as<uint64_t>(), nullptr, keys_alt.as<uint64_t>(), nullptr, n, shift,
+				    h->hist.as<uint32_t>(), ntiles);
+			HIP_LAUNCH_CHECK(h);
+		}
+		std::swap(keys, keys_alt);
+		if (kv)
+			std::swap(*vals, *vals_alt);
+	}
+	return XTN_OK;
+}
+
+int read_ctrs(xtn_handle* h, RunCtrs* out) {
+	HIP_TRY(h, hipMemcpyAsync(out, h->ctrs.p, sizeof(RunCtrs), hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	return XTN_OK;
+}
+
+int bits_for(int64_t maxval) {
+	int b = 1;
+	while (b < 62 && (1ll << b) <= maxval)
+		++b;
+	return b;
+}
+
+double now_ms() {
+	using namespace std::chrono;
+	return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ----------------------------------------------------------- NN pipeline --
+
+// Runs the candidate search and verification over `nseq` ids. seq_map (device,
+// nullable) maps an id to its input row. Leaves sorted (x, y, distance)
+// triples in out_pairs/out_dist and their count in h->out_len.
+int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
+	hipStream_t s = h->stream;
+	const int d = h->cfg.distance;
+	const int measure = h->cfg.measure;
+	xtn_stats& st = h->stats;
+	const double t0 = now_ms();
+	h->out_len = 0;
+	st.n_sequences = nseq;
+	if (nseq < 2) {
+		HIP_TRY(h, hipStreamSynchronize(s));
+		st.ms_total = now_ms() - t0;
+		return XTN_OK;
+	}
+	const int idb = bits_for(nseq - 1);
+	if (2 * idb > 62)
+		return fail(h, XTN_ERR_CAPACITY, "too many sequences (%d)", nseq);
+
+	// pack + variant offsets
+	TRY(ensure(h, h->pk, sizeof(uint64_t) * 2 * (size_t)nseq));
+	TRY(ensure(h, h->voff, sizeof(uint64_t) * ((size_t)nseq + 1)));
+	TRY(ensure(h, h->ctrs, sizeof(RunCtrs)));
+	TRY(ensure(h, h->scalar, 64));
+	{
+		ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
+		k_pack<<<grid_for(nseq, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
+		                                            seq_map, nseq, d, h->pk.as<uint64_t>(),
+		                                            h->voff.as<uint64_t>());
+		HIP_LAUNCH_CHECK(h);
+	}
+	TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), nseq,
+	                        h->voff.as<uint64_t>() + nseq));
+	uint64_t V = 0;
+	HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + nseq, sizeof(uint64_t),
+	                          hipMemcpyDeviceToHost, s));
+	HIP_TRY(h, hipStreamSynchronize(s));
+	st.n_variants = (int64_t)V;
+	const double t_gen0 = now_ms();
+
+	// chunk plan: records of one chunk (plus sort buffer, run lists and
+	// histograms, ~32 B per record) must fit the budget
+	int64_t budget = h->cfg.max_records_per_chunk;
+	if (budget <= 0) {
+		size_t fr = 0, tot = 0;
+		HIP_TRY(h, hipMemGetInfo(&fr, &tot));
+		budget = (int64_t)((double)fr * 0.6 / 32.0);
+		budget = std::min<int64_t>(budget, (int64_t)1 << 31);
+	}
+	budget = std::max<int64_t>(budget, 1 << 16);
+	int pbits = 0;
+	while (((int64_t)V >> pbits) > budget && pbits < 12)
+		++pbits;
+	// partitions are hash buckets: give them 25% headroom over the mean
+	while (pbits < 12 && ((int64_t)V >> pbits) + ((int64_t)V >> (pbits + 2)) > budget && pbits > 0)
+		++pbits;
+	const int nparts = 1 << pbits;
+	const int hash_bits = 64 - idb;
+	const int K = std::min(32, hash_bits - pbits);
+	if (K < 8)
+		return fail(h, XTN_ERR_CAPACITY, "too few key bits (%d sequences, %d chunks)", nseq, nparts);
+	const int rs = 64 - pbits - K;
+	st.n_chunks = nparts;
+
+	GenArgs ga{};
+	ga.pk = h->pk.as<uint64_t>();
+	ga.voff = h->voff.as<uint64_t>();
+	ga.n = nseq;
+	ga.distance = d;
+	ga.idb = idb;
+	ga.part_bits = pbits;
+	ga.V = (int64_t)V;
+	const unsigned gen_grid = grid_for((int64_t)V, GEN_TILE);
+
+	std::vector<unsigned long long> part_count(nparts, 0);
+	int64_t cap_recs = (int64_t)V;
+	if (nparts > 1) {
+		TRY(ensure(h, h->part_hist, sizeof(unsigned long long) * nparts));
+		k_fill_u64<<<grid_for(nparts, 256), 256, 0, s>>>(h->part_hist.as<unsigned long long>(), nparts, 0ull);
+		ga.part_hist = h->part_hist.as<unsigned long long>();
+		{
+			ProfScope ps(h, "gen_count", (double)nseq * 16.0);
+			k_gen<GEN_HIST><<<gen_grid, GEN_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		}
+		HIP_TRY(h, hipMemcpyAsync(part_count.data(), h->part_hist.p, sizeof(unsigned long long) * nparts,
+		                          hipMemcpyDeviceToHost, s));
+		HIP_TRY(h, hipStreamSynchronize(s));
+		cap_recs = 0;
+		for (auto c : part_count)
+			cap_recs = std::max<int64_t>(cap_recs, (int64_t)c);
+	} else {
+		part_count[0] = V;
+	}
+	if (cap_recs > (int64_t)UINT32_MAX - 1)
+		return fail(h, XTN_ERR_CAPACITY, "chunk of %lld records exceeds 2^32", (long long)cap_recs);
+	TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)cap_recs));
+	TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)cap_recs));
+	TRY(ensure(h, h->runs_small, sizeof(Run) * (size_t)(cap_recs / 2 + 1)));
+	TRY(ensure(h, h->runs_big, sizeof(Run) * (size_t)(cap_recs / (SMALL_RUN_MAX + 1) + 1)));
+
+	int64_t n_pairs = 0;
+	double ms_gen = 0, ms_sort = 0, ms_pairs = 0;
+	for (int part = 0; part < nparts; ++part) {
+		const int64_t nrec = (int64_t)part_count[part];
+		if (nrec < 2)
+			continue;
+		double ta = now_ms();
+		ga.out = h->recs.as<uint64_t>();
+		ga.cap = cap_recs;
+		if (nparts == 1) {
+			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0);
+			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		} else {
+			ga.part_id = (uint32_t)part;
+			ga.append = &h->ctrs.as<RunCtrs>()->append;
+			k_fill_u64<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(h->ctrs.p), 8, 0ull);
+			ProfScope ps(h, "gen", (double)nrec * 8.0 + (double)nseq * 24.0);
+			k_gen<GEN_APPEND><<<gen_grid, GEN_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		}
+		HIP_TRY(h, hipStreamSynchronize(s));
+		double tb = now_ms();
+		ms_gen += tb - ta;
+
+		TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, rs, rs + K, "sort_rec"));
+		HIP_TRY(h, hipStreamSynchronize(s));
+		double tc = now_ms();
+		ms_sort += tc - tb;
+
+		// runs
+		k_fill_u64<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(h->ctrs.p), 8, 0ull);
+		{
+			ProfScope ps(h, "runs", (double)nrec * 8.0);
+			k_runs<<<grid_for(nrec, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), nrec, rs,
+			                                            h->runs_small.as<Run>(), cap_recs / 2 + 1,
+			                                            h->runs_big.as<Run>(),
+			                                            cap_recs / (SMALL_RUN_MAX + 1) + 1,
+			                                            h->ctrs.as<RunCtrs>());
+			HIP_LAUNCH_CHECK(h);
+		}
+		RunCtrs rc{};
+		TRY(read_ctrs(h, &rc));
+		if ((int64_t)rc.n_small > cap_recs / 2 + 1 || (int64_t)rc.n_big > cap_recs / (SMALL_RUN_MAX + 1) + 1)
+			return fail(h, XTN_ERR_HIP, "run list overflow (%llu small, %llu big)",
+			            (unsigned long long)rc.n_small, (unsigned long long)rc.n_big);
+		st.n_runs += (int64_t)(rc.n_small + rc.n_big);
+		const uint64_t bound = rc.pairs_small + rc.pairs_big;
+		if ((uint64_t)n_pairs + bound > (uint64_t)INT32_MAX)
+			return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31 (%llu)",
+			            (unsigned long long)((uint64_t)n_pairs + bound));
+		const int64_t need = n_pairs + (int64_t)bound;
+		if ((size_t)need * sizeof(uint64_t) > h->pairs.cap)
+			TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)std::max<int64_t>(need, 2 * n_pairs),
+			              sizeof(uint64_t) * (size_t)n_pairs));
+		// append cursor starts after the pairs of earlier chunks
+		k_fill_u64<<<1, 256, 0, s>>>(&h->ctrs.as<RunCtrs>()->n_cand, 1, (unsigned long long)n_pairs);
+		const int64_t pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
+		if (rc.n_small) {
+			ProfScope ps(h, "pairs_small", (double)rc.n_small * 8.0);
+			k_pairs_small<<<grid_for((int64_t)rc.n_small, 256), 256, 0, s>>>(
+			    h->recs.as<uint64_t>(), h->runs_small.as<Run>(), (int64_t)rc.n_small, idb,
+			    h->pairs.as<uint64_t>(), pcap, h->ctrs.as<RunCtrs>());
+			HIP_LAUNCH_CHECK(h);
+		}
+		if (rc.n_big) {
+			TRY(ensure(h, h->big_pre, sizeof(uint64_t) * ((size_t)rc.n_big + 1)));
+			k_big_counts<<<grid_for((int64_t)rc.n_big, 256), 256, 0, s>>>(
+			    h->runs_big.as<Run>(), (int64_t)rc.n_big, h->big_pre.as<uint64_t>());
+			HIP_LAUNCH_CHECK(h);
+			TRY(scan_excl<uint64_t>(h, h->big_pre.as<uint64_t>(), h->big_pre.as<uint64_t>(),
+			                        (int64_t)rc.n_big, h->big_pre.as<uint64_t>() + rc.n_big));
+			const uint64_t tb_pairs = rc.pairs_big;
+			const unsigned g = (unsigned)std::min<uint64_t>((tb_pairs + 255) / 256, 16384);
+			ProfScope ps(h, "pairs_big", (double)tb_pairs * 16.0);
+			k_pairs_big<<<g > 0 ? g : 1, 256, 0, s>>>(h->recs.as<uint64_t>(), h->runs_big.as<Run>(),
+			                                          h->big_pre.as<uint64_t>(), (int64_t)rc.n_big, idb,
+			                                          h->pairs.as<uint64_t>(), pcap, h->ctrs.as<RunCtrs>());
+			HIP_LAUNCH_CHECK(h);
+		}
+		TRY(read_ctrs(h, &rc));
+		if ((int64_t)rc.n_cand > pcap)
+			return fail(h, XTN_ERR_HIP, "pair buffer overflow (%llu > %lld)",
+			            (unsigned long long)rc.n_cand, (long long)pcap);
+		n_pairs = (int64_t)rc.n_cand;
+		ms_pairs += now_ms() - tc;
+	}
+	st.n_candidates = n_pairs;
+	st.ms_generate = ms_gen;
+	st.ms_sort = ms_sort;
+	st.ms_pairs = ms_pairs;
+
+	// sort + unique + verify
+	const double tv = now_ms();
+	if (n_pairs > 0) {
+		TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)n_pairs));
+		TRY(radix_sort(h, h->pairs, h->pairs_alt, nullptr, nullptr, n_pairs, 0, 2 * idb, "sort_pairs"));
+		TRY(ensure(h, h->flags, sizeof(uint32_t) * (size_t)n_pairs));
+		TRY(ensure(h, h->pos, sizeof(uint32_t) * ((size_t)n_pairs + 1)));
+		TRY(ensure(h, h->dist, (size_t)n_pairs));
+		k_fill_u64<<<1, 256, 0, s>>>(&h->ctrs.as<RunCtrs>()->n_unique, 1, 0ull);
+		{
+			ProfScope ps(h, "verify", (double)n_pairs * 13.0);
+			k_verify<<<grid_for(n_pairs, 256), 256, 0, s>>>(
+			    h->pairs.as<uint64_t>(), n_pairs, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
+			    seq_map, d, measure, h->flags.as<uint32_t>(), h->dist.as<int8_t>(), h->ctrs.as<RunCtrs>());
+			HIP_LAUNCH_CHECK(h);
+		}
+		TRY(scan_excl<uint32_t>(h, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), n_pairs,
+		                        h->pos.as<uint32_t>() + n_pairs));
+		uint32_t nout = 0;
+		HIP_TRY(h, hipMemcpyAsync(&nout, h->pos.as<uint32_t>() + n_pairs, sizeof(uint32_t),
+		                          hipMemcpyDeviceToHost, s));
+		RunCtrs rc{};
+		TRY(read_ctrs(h, &rc));
+		st.n_unique = (int64_t)rc.n_unique;
+		TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
+		TRY(ensure(h, h->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
+		{
+			ProfScope ps(h, "emit", (double)n_pairs * 9.0 + (double)nout * 9.0);
+			k_emit_nn<<<grid_for(n_pairs, 256), 256, 0, s>>>(
+			    h->pairs.as<uint64_t>(), n_pairs, idb, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(),
+			    h->dist.as<int8_t>(), h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
+			HIP_LAUNCH_CHECK(h);
+		}
+		HIP_TRY(h, hipStreamSynchronize(s));
+		h->out_len = nout;
+	} else {
+		HIP_TRY(h, hipStreamSynchronize(s));
+	}
+	st.n_output = h->out_len;
+	st.ms_verify = now_ms() - tv;
+	st.ms_total = now_ms() - t0;
+	(void)t_gen0;
+	collect_profile(h);
+	return XTN_OK;
+}
+
+bool valid_residue(unsigned char c) {
+	if (c < 'A' || c > 'Y')
+		return false;
+	return !(c == 'B' || c == 'J' || c == 'O' || c == 'U' || c == 'X');
+}
+
+void print_verbose(xtn_handle* h, const char* mode) {
+	if (!h->cfg.verbose)
+		return;
+	const xtn_stats& st = h->stats;
+	printf("mode %s: sequences %lld, variants %lld, chunks %lld, runs %lld, candidates %lld, "
+	       "unique %lld, output %lld\n",
+	       mode, (long long)st.n_sequences, (long long)st.n_variants, (long long)st.n_chunks,
+	       (long long)st.n_runs, (long long)st.n_candidates, (long long)st.n_unique,
+	       (long long)st.n_output);
+	printf("device ms: total %.3f generate %.3f sort %.3f pairs %.3f verify %.3f output %.3f\n",
+	       st.ms_total, st.ms_generate, st.ms_sort, st.ms_pairs, st.ms_verify, st.ms_output);
+}
+
+int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, void* user) {
+	(void)info;
+	(void)callback;
+	(void)user;
+	return fail(h, XTN_ERR_STATE, "overlap mode not built yet");
+}
+
+}  // namespace
+
+// =============================================================== C ABI ==
+
+extern "C" {
+
+void xtn_config_init(xtn_config* cfg) {
+	if (!cfg)
+		return;
+	memset(cfg, 0, sizeof(*cfg));
+	cfg->distance = 1;
+	cfg->measure = XTN_LEVENSHTEIN;
+}
+
+static int check_config(xtn_handle* h, const xtn_config* cfg) {
+	if (cfg->distance < 1)
+		return fail(h, XTN_ERR_ARG, "distance must be a valid number larger than 0");
+	if (cfg->distance > 127)
+		return fail(h, XTN_ERR_ARG, "distance above 127 is not supported");
+	if (cfg->measure != XTN_LEVENSHTEIN && cfg->measure != XTN_HAMMING)
+		return fail(h, XTN_ERR_ARG, "invalid measure option");
+	return XTN_OK;
+}
+
+int xtn_open(xtn_handle** out, const xtn_config* cfg) {
+	if (!out)
+		return XTN_ERR_ARG;
+	*out = nullptr;
+	xtn_config c;
+	xtn_config_init(&c);
+	if (cfg)
+		c = *cfg;
+	if (check_config(nullptr, &c) != XTN_OK)
+		return XTN_ERR_ARG;
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+		(void)hipGetLastError();
+		return XTN_ERR_HIP;
+	}
+	if (c.device < 0 || c.device >= ndev)
+		return XTN_ERR_ARG;
+	if (hipSetDevice(c.device) != hipSuccess)
+		return XTN_ERR_HIP;
+	xtn_handle* h = new xtn_handle();
+	h->cfg = c;
+	memset(&h->stats, 0, sizeof(h->stats));
+	if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+	    hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+		delete h;
+		return XTN_ERR_HIP;
+	}
+	*out = h;
+	return XTN_OK;
+}
+
+void xtn_close(xtn_handle* h) {
+	if (!h)
+		return;
+	(void)hipSetDevice(h->cfg.device);
+	(void)hipStreamSynchronize(h->stream);
+	DBuf* bufs[] = {&h->chars, &h->offs, &h->pk, &h->voff, &h->recs, &h->recs_alt, &h->hist,
+	                &h->scan_tmp, &h->runs_small, &h->runs_big, &h->big_pre, &h->pairs,
+	                &h->pairs_alt, &h->flags, &h->pos, &h->dist, &h->out_pairs, &h->out_dist,
+	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
+	                &h->ov_vals_alt, &h->ov_info, &h->ov_leader, &h->ov_ustart, &h->ov_umap,
+	                &h->ov_cnt, &h->ov_matrix, &h->ov_present};
+	for (DBuf* b : bufs)
+		release(*b);
+	for (auto& pe : h->pending) {
+		(void)hipEventDestroy(pe.start);
+		(void)hipEventDestroy(pe.stop);
+	}
+	for (hipEvent_t e : h->event_pool)
+		(void)hipEventDestroy(e);
+	(void)hipStreamDestroy(h->stream);
+	(void)hipStreamDestroy(h->copy_stream);
+	delete h;
+}
+
+int xtn_set_config(xtn_handle* h, const xtn_config* cfg) {
+	if (!h || !cfg)
+		return XTN_ERR_ARG;
+	TRY(check_config(h, cfg));
+	if (cfg->device != h->cfg.device)
+		return fail(h, XTN_ERR_ARG, "the device of an open handle cannot change");
+	h->cfg = *cfg;
+	return XTN_OK;
+}
+
+int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets, int32_t n) {
+	if (!h || n < 0 || !offsets || (n > 0 && !chars && offsets[n] > 0))
+		return h ? fail(h, XTN_ERR_ARG, "invalid sequence arguments") : XTN_ERR_ARG;
+	if (offsets[0] != 0)
+		return fail(h, XTN_ERR_ARG, "offsets[0] must be 0");
+	for (int32_t i = 0; i < n; ++i)
+		if (offsets[i + 1] < offsets[i])
+			return fail(h, XTN_ERR_ARG, "offsets must be non-decreasing (row %d)", i);
+	const uint32_t total = offsets[n];
+	for (uint32_t k = 0; k < total; ++k)
+		if (!valid_residue((unsigned char)chars[k]))
+			return fail(h, XTN_ERROR, "invalid char");
+	HIP_TRY(h, hipSetDevice(h->cfg.device));
+	TRY(ensure(h, h->chars, (size_t)total + 16));
+	TRY(ensure(h, h->offs, sizeof(uint32_t) * ((size_t)n + 1)));
+	if (total)
+		HIP_TRY(h, hipMemcpyAsync(h->chars.p, chars, total, hipMemcpyHostToDevice, h->stream));
+	HIP_TRY(h, hipMemcpyAsync(h->offs.p, offsets, sizeof(uint32_t) * ((size_t)n + 1),
+	                          hipMemcpyHostToDevice, h->stream));
+	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	h->n = n;
+	h->n_chars = total;
+	h->out_len = 0;
+	return XTN_OK;
+}
+
+int xtn_run_nn_resident(xtn_handle* h, int64_t* out_len) {
+	if (!h)
+		return XTN_ERR_ARG;
+	if (!h->offs.p)
+		return fail(h, XTN_ERR_STATE, "no sequences set");
+	HIP_TRY(h, hipSetDevice(h->cfg.device));
+	memset(&h->stats, 0, sizeof(h->stats));
+	TRY(nn_pipeline(h, h->n, nullptr));
+	if (out_len)
+		*out_len = h->out_len;
+	print_verbose(h, "nn");
+	return XTN_OK;
+}
+
+int xtn_copy_nn_output(xtn_handle* h, xtn_int2* pairs, int8_t* distances, int64_t cap) {
+	if (!h || (cap > 0 && (!pairs || !distances)))
+		return XTN_ERR_ARG;
+	const int64_t n = std::min<int64_t>(cap, h->out_len);
+	if (n <= 0)
+		return XTN_OK;
+	HIP_TRY(h, hipMemcpyAsync(pairs, h->out_pairs.p, sizeof(xtn_int2) * (size_t)n,
+	                          hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(h, hipMemcpyAsync(distances, h->out_dist.p, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	return XTN_OK;
+}
+
+int xtn_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, void* user) {
+	if (!h)
+		return XTN_ERR_ARG;
+	if (!h->offs.p)
+		return fail(h, XTN_ERR_STATE, "no sequences set");
+	HIP_TRY(h, hipSetDevice(h->cfg.device));
+	memset(&h->stats, 0, sizeof(h->stats));
+	if (info)
+		return overlap_perform(h, info, callback, user);
+
+	TRY(nn_pipeline(h, h->n, nullptr));
+	print_verbose(h, "nn");
+	const double t0 = now_ms();
+	const int64_t n = h->out_len;
+	int64_t chunk = h->cfg.output_chunk_pairs > 0 ? h->cfg.output_chunk_pairs : ((int64_t)1 << 22);
+	if (n == 0) {
+		h->stats.ms_output = now_ms() - t0;
+		return XTN_OK;
+	}
+	chunk = std::min(chunk, n);
+	// double-buffered pinned staging: chunk k+1 is copied on the side stream
+	// while the callback consumes chunk k
+	xtn_int2* hp[2] = {nullptr, nullptr};
+	int8_t* hd[2] = {nullptr, nullptr};
+	hipEvent_t ready[2] = {nullptr, nullptr};
+	int rc = XTN_OK;
+	for (int b = 0; b < 2 && rc == XTN_OK; ++b) {
+		if (hipHostMalloc((void**)&hp[b], sizeof(xtn_int2) * (size_t)chunk, hipHostMallocDefault) != hipSuccess ||
+		    hipHostMalloc((void**)&hd[b], (size_t)chunk, hipHostMallocDefault) != hipSuccess ||
+		    hipEventCreateWithFlags(&ready[b], hipEventDisableTiming) != hipSuccess)
+			rc = fail(h, XTN_ERR_NOMEM, "pinned staging allocation failed");
+	}
+	auto issue = [&](int64_t start, int b) -> int {
+		const int64_t len = std::min(chunk, n - start);
+		HIP_TRY(h, hipMemcpyAsync(hp[b], h->out_pairs.as<int2>() + start, sizeof(xtn_int2) * (size_t)len,
+		                          hipMemcpyDeviceToHost, h->copy_stream));
+		HIP_TRY(h, hipMemcpyAsync(hd[b], h->out_dist.as<int8_t>() + start, (size_t)len,
+		                          hipMemcpyDeviceToHost, h->copy_stream));
+		HIP_TRY(h, hipEventRecord(ready[b], h->copy_stream));
+		return XTN_OK;
+	};
+	if (rc == XTN_OK)
+		rc = issue(0, 0);
+	for (int64_t start = 0, k = 0; rc == XTN_OK && start < n; start += chunk, ++k) {
+		const int b = (int)(k & 1);
+		if (hipEventSynchronize(ready[b]) != hipSuccess) {
+			rc = fail(h, XTN_ERR_HIP, "output copy failed");
+			break;
+		}
+		if (start + chunk < n) {
+			rc = issue(start + chunk, b ^ 1);
+			if (rc != XTN_OK)
+				break;
+		}
+		xtn_output out;
+		out.index_pairs = hp[b];
+		out.pairwise_distances = hd[b];
+		out.pairwise_frequencies = nullptr;
+		out.len = std::min(chunk, n - start);
+		if (callback)
+			callback(user, &out);
+	}
+	(void)hipStreamSynchronize(h->copy_stream);
+	for (int b = 0; b < 2; ++b) {
+		if (hp[b])
+			(void)hipHostFree(hp[b]);
+		if (hd[b])
+			(void)hipHostFree(hd[b]);
+		if (ready[b])
+			(void)hipEventDestroy(ready[b]);
+	}
+	h->stats.ms_output = now_ms() - t0;
+	return rc;
+}
+
+int xtn_get_stats(xtn_handle* h, xtn_stats* out) {
+	if (!h || !out)
+		return XTN_ERR_ARG;
+	*out = h->stats;
+	return XTN_OK;
+}
+
+int xtn_kernel_profile(xtn_handle* h, const char* kernel, int64_t* launches, double* ms, double* bytes) {
+	if (!h || !kernel)
+		return XTN_ERR_ARG;
+	auto it = h->prof.find(kernel);
+	ProfAcc acc;
+	if (it != h->prof.end())
+		acc = it->second;
+	if (launches)
+		*launches = acc.launches;
+	if (ms)
+		*ms = acc.ms;
+	if (bytes)
+		*bytes = acc.bytes;
+	return XTN_OK;
+}
+
+int xtn_kernel_profile_reset(xtn_handle* h) {
+	if (!h)
+		return XTN_ERR_ARG;
+	h->prof.clear();
+	return XTN_OK;
+}
+
+const char* xtn_last_error(xtn_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+const char* xtn_status_string(int status) {
+	switch (status) {
+	case XTN_OK: return "ok";
+	case XTN_ERROR: return "invalid input";
+	case XTN_EXIT: return "exit";
+	case XTN_ERR_ARG: return "invalid argument";
+	case XTN_ERR_HIP: return "HIP runtime error";
+	case XTN_ERR_NOMEM: return "out of memory";
+	case XTN_ERR_CAPACITY: return "capacity exceeded";
+	case XTN_ERR_STATE: return "invalid call order";
+	default: return "unknown status";
+	}
+}
+
+const char* xtn_version(void) { return "2.0.0-mi355x"; }
+
+}  // extern "C"

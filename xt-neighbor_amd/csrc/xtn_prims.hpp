@@ -1,0 +1,302 @@
+// xtn_prims.hpp -- device-wide primitives for gfx950 (wave64): exclusive scan
+// and a stable LSD radix sort of 64-bit keys (optionally carrying 32-bit
+// values). They replace the CUB facade of the reference (cub.cu:80-285:
+// DeviceScan, DeviceMergeSort, DeviceRunLengthEncode, DeviceSelect).
+//
+// Radix sort: reduce-then-scan per 8-bit digit pass.
+//   k_radix_hist    one 4096-key tile per workgroup, LDS digit histogram,
+//                   counts written digit-major so one exclusive scan yields
+//                   every (digit, tile) output base.
+//   k_radix_scatter the tile is ranked stably inside each wave with eight
+//                   ballots (the lanes holding the same digit), wave-private
+//                   LDS counters carry the rank across the wave's 16 rounds,
+//                   the tile is reordered by digit in LDS and written out in
+//                   digit runs (coalesced).
+// No kernel here spins or waits on another workgroup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xtn {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+	const int lane = lane_id();
+	return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+	const int lane = lane_id();
+#pragma unroll
+	for (int o = 1; o < WAVE; o <<= 1) {
+		T u = __shfl_up(v, o, WAVE);
+		if (lane >= o)
+			v += u;
+	}
+	return v;
+}
+
+// Exclusive scan across a workgroup of NT threads. s_tmp needs NT/WAVE+1
+// slots. Returns the thread's exclusive prefix; *total gets the sum.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* s_tmp, T* total) {
+	constexpr int NW = NT / WAVE;
+	const int lane = lane_id();
+	const int wid = threadIdx.x / WAVE;
+	T inc = wave_incl_scan(v);
+	if (lane == WAVE - 1)
+		s_tmp[wid] = inc;
+	__syncthreads();
+	if (wid == 0) {
+		T w = lane < NW ? s_tmp[lane] : T(0);
+		T wi = wave_incl_scan(w);
+		if (lane < NW)
+			s_tmp[lane] = wi - w;
+		if (lane == NW - 1)
+			s_tmp[NW] = wi;
+	}
+	__syncthreads();
+	T r = s_tmp[wid] + inc - v;
+	if (total)
+		*total = s_tmp[NW];
+	__syncthreads();
+	return r;
+}
+
+// Appends `count` items per lane to a global counter with one atomic per wave.
+// Returns this lane's first slot. Every lane of the wave must call it.
+__device__ __forceinline__ unsigned long long wave_append(unsigned long long* ctr, uint32_t count) {
+	const int lane = lane_id();
+	uint32_t inc = wave_incl_scan(count);
+	uint32_t total = __shfl(inc, WAVE - 1, WAVE);
+	unsigned long long base = 0;
+	if (lane == 0 && total > 0)
+		base = atomicAdd(ctr, (unsigned long long)total);
+	base = __shfl(base, 0, WAVE);
+	return base + (inc - count);
+}
+
+// ---------------------------------------------------------------- scan --
+
+constexpr int SC_NT = 256;
+constexpr int SC_IT = 8;
+constexpr int SC_TILE = SC_NT * SC_IT;
+
+template <typename T>
+__global__ __launch_bounds__(SC_NT) void k_scan_sums(const T* __restrict__ in, int64_t n,
+                                                     T* __restrict__ sums) {
+	__shared__ T s_tmp[SC_NT / WAVE + 1];
+	const int64_t base = (int64_t)blockIdx.x * SC_TILE;
+	T acc = 0;
+#pragma unroll
+	for (int i = 0; i < SC_IT; ++i) {
+		int64_t idx = base + (int64_t)i * SC_NT + threadIdx.x;
+		if (idx < n)
+			acc += in[idx];
+	}
+	T tot;
+	block_excl_scan<SC_NT>(acc, s_tmp, &tot);
+	if (threadIdx.x == 0)
+		sums[blockIdx.x] = tot;
+}
+
+// exclusive scan of data[0..n) in place by a single workgroup
+template <typename T>
+__global__ __launch_bounds__(SC_NT) void k_scan_single(T* __restrict__ data, int64_t n,
+                                                       T* __restrict__ total) {
+	__shared__ T s_tmp[SC_NT / WAVE + 1];
+	T carry = 0;
+	for (int64_t base = 0; base < n; base += SC_TILE) {
+		T vals[SC_IT];
+		T sum = 0;
+#pragma unroll
+		for (int i = 0; i < SC_IT; ++i) {
+			int64_t idx = base + (int64_t)threadIdx.x * SC_IT + i;
+			vals[i] = idx < n ? data[idx] : T(0);
+			sum += vals[i];
+		}
+		T tot;
+		T run = carry + block_excl_scan<SC_NT>(sum, s_tmp, &tot);
+#pragma unroll
+		for (int i = 0; i < SC_IT; ++i) {
+			int64_t idx = base + (int64_t)threadIdx.x * SC_IT + i;
+			if (idx < n)
+				data[idx] = run;
+			run += vals[i];
+		}
+		carry += tot;
+	}
+	if (threadIdx.x == 0 && total)
+		*total = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SC_NT) void k_scan_apply(const T* in, T* out, int64_t n,
+                                                      const T* __restrict__ sums) {
+	__shared__ T s_tmp[SC_NT / WAVE + 1];
+	const int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_IT;
+	T vals[SC_IT];
+	T sum = 0;
+#pragma unroll
+	for (int i = 0; i < SC_IT; ++i) {
+		int64_t idx = base + i;
+		vals[i] = idx < n ? in[idx] : T(0);
+		sum += vals[i];
+	}
+	T run = sums[blockIdx.x] + block_excl_scan<SC_NT>(sum, s_tmp, (T*)nullptr);
+#pragma unroll
+	for (int i = 0; i < SC_IT; ++i) {
+		int64_t idx = base + i;
+		if (idx < n)
+			out[idx] = run;
+		run += vals[i];
+	}
+}
+
+// ---------------------------------------------------------- radix sort --
+
+constexpr int RS_NT = 256;
+constexpr int RS_IT = 16;
+constexpr int RS_TILE = RS_NT * RS_IT;  // 4096 keys per workgroup
+constexpr int RS_BINS = 256;
+constexpr int RS_WAVES = RS_NT / WAVE;
+constexpr int RS_SEG = RS_IT * WAVE;  // keys per wave segment
+
+static_assert(RS_NT == RS_BINS, "one thread per digit in the prefix step");
+
+__global__ __launch_bounds__(RS_NT) void k_radix_hist(const uint64_t* __restrict__ keys, int64_t n,
+                                                      int shift, uint32_t* __restrict__ hist,
+                                                      int64_t ntiles) {
+	__shared__ uint32_t s_h[RS_WAVES][RS_BINS];
+	for (int i = threadIdx.x; i < RS_WAVES * RS_BINS; i += RS_NT)
+		(&s_h[0][0])[i] = 0;
+	__syncthreads();
+	const int w = threadIdx.x / WAVE;
+	const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+	for (int i = 0; i < RS_IT; ++i) {
+		int64_t idx = base + (int64_t)i * RS_NT + threadIdx.x;
+		if (idx < n) {
+			uint32_t d = (uint32_t)(keys[idx] >> shift) & 0xFFu;
+			atomicAdd(&s_h[w][d], 1u);
+		}
+	}
+	__syncthreads();
+	const int d = threadIdx.x;
+	uint32_t c = 0;
+#pragma unroll
+	for (int ww = 0; ww < RS_WAVES; ++ww)
+		c += s_h[ww][d];
+	hist[(int64_t)d * ntiles + blockIdx.x] = c;
+}
+
+// lanes of the wave whose digit equals this lane's (among valid lanes)
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {
+	uint64_t peers = __ballot(valid);
+#pragma unroll
+	for (int b = 0; b < 8; ++b) {
+		const bool bit = (d >> b) & 1u;
+		const uint64_t m = __ballot(bit);
+		peers &= bit ? m : ~m;
+	}
+	return peers;
+}
+
+template <bool KV>
+__global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
+                                                         const uint32_t* __restrict__ vals_in,
+                                                         uint64_t* __restrict__ keys_out,
+                                                         uint32_t* __restrict__ vals_out, int64_t n,
+                                                         int shift, const uint32_t* __restrict__ offs,
+                                                         int64_t ntiles) {
+	__shared__ uint64_t s_keys[RS_TILE];
+	__shared__ uint32_t s_vals[KV ? RS_TILE : 1];
+	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+	__shared__ uint32_t s_dpre[RS_BINS];
+	__shared__ uint32_t s_gbase[RS_BINS];
+	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+
+	const int t = threadIdx.x;
+	const int lane = lane_id();
+	const int w = t / WAVE;
+	for (int i = t; i < RS_WAVES * RS_BINS; i += RS_NT)
+		(&s_cnt[0][0])[i] = 0;
+	__syncthreads();
+
+	const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+	const int64_t seg = base + (int64_t)w * RS_SEG;
+	const uint64_t lt = lanemask_lt();
+	uint64_t k[RS_IT];
+	uint32_t v[RS_IT];
+	uint32_t rk[RS_IT];
+
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		const int64_t idx = seg + (int64_t)r * WAVE + lane;
+		const bool valid = idx < n;
+		k[r] = valid ? keys_in[idx] : 0ull;
+		if (KV)
+			v[r] = valid ? vals_in[idx] : 0u;
+		const uint32_t d = (uint32_t)(k[r] >> shift) & 0xFFu;
+		const uint64_t peers = match_digit8(d, valid);
+		const uint32_t before = s_cnt[w][d];
+		const uint32_t rank = before + (uint32_t)__popcll(peers & lt);
+		if (valid && (__ffsll((unsigned long long)peers) - 1) == lane)
+			s_cnt[w][d] = before + (uint32_t)__popcll(peers);
+		rk[r] = valid ? ((rank << 8) | d) : 0xFFFFFFFFu;
+	}
+	__syncthreads();
+
+	{
+		const int d = t;
+		uint32_t c[RS_WAVES];
+		uint32_t tot = 0;
+#pragma unroll
+		for (int ww = 0; ww < RS_WAVES; ++ww) {
+			c[ww] = s_cnt[ww][d];
+			tot += c[ww];
+		}
+		const uint32_t ex = block_excl_scan<RS_NT>(tot, s_scan, (uint32_t*)nullptr);
+		s_dpre[d] = ex;
+		uint32_t run = ex;
+#pragma unroll
+		for (int ww = 0; ww < RS_WAVES; ++ww) {
+			s_cnt[ww][d] = run;
+			run += c[ww];
+		}
+		s_gbase[d] = offs[(int64_t)d * ntiles + blockIdx.x];
+	}
+	__syncthreads();
+
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		if (rk[r] != 0xFFFFFFFFu) {
+			const uint32_t d = rk[r] & 0xFFu;
+			const uint32_t lp = s_cnt[w][d] + (rk[r] >> 8);
+			s_keys[lp] = k[r];
+			if (KV)
+				s_vals[lp] = v[r];
+		}
+	}
+	__syncthreads();
+
+	const int64_t rem = n - base;
+	const int tile_n = rem < RS_TILE ? (int)rem : RS_TILE;
+	for (int i = t; i < tile_n; i += RS_NT) {
+		const uint64_t key = s_keys[i];
+		const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+		const uint32_t g = s_gbase[d] + ((uint32_t)i - s_dpre[d]);
+		if ((int64_t)g < n) {
+			keys_out[g] = key;
+			if (KV)
+				vals_out[g] = s_vals[i];
+		}
+	}
+}
+
+}  // namespace xtn
