@@ -67,6 +67,19 @@ def test_large_groups_use_flattened_pairs(engine):
     assert np.array_equal(got, oracle.nn_brute(chars, offsets, 2, 0))
 
 
+@pytest.mark.parametrize("copies", [3, 4, 7])
+def test_runs_across_tile_boundaries(engine, copies):
+    # every string repeated: thousands of runs of 3..10 records, many of
+    # them straddling the 4096-record tiles of the run detector
+    base = xtn.decode(*xtn.synth_nn(6, 6000 // copies, planted=0.3))
+    seqs = [s for s in base for _ in range(copies)]
+    chars, offsets = xtn.encode(seqs)
+    for d in (1, 2):
+        got, _ = run(engine, chars=chars, offsets=offsets, d=d)
+        port, _ = oracle.nn_port(chars, offsets, d, 0)
+        assert np.array_equal(got, port), d
+
+
 @pytest.mark.parametrize("chunk", [1 << 16, 70000])
 def test_multi_chunk_equals_single(engine, chunk):
     chars, offsets = xtn.synth_nn(3, 20000, planted=0.25, max_edits=2)

@@ -116,16 +116,34 @@ __device__ __forceinline__ uint64_t hash_variant(u128 v) {
 	return mix64(lo ^ (hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull);
 }
 
+// del_slot with precomputed masks: hm[pos] keeps slots 0..pos-1, lm[pos]
+// keeps slots pos+1..17 (moved up by one slot)
+__device__ __forceinline__ u128 del_slot_t(u128 p, int pos, const u128* __restrict__ hm,
+                                           const u128* __restrict__ lm) {
+	return (p & hm[pos]) | ((p & lm[pos]) << 5);
+}
+
+__device__ __forceinline__ void init_slot_masks(u128* hm, u128* lm, int t, int nt) {
+	for (int pos = t; pos < MAX_PREFIX; pos += nt) {
+		const int low_bits = 5 * (MAX_PREFIX - 1 - pos);
+		const u128 one = 1;
+		lm[pos] = (one << low_bits) - 1;
+		hm[pos] = ~((one << (low_bits + 5)) - 1);
+	}
+}
+
 // the k-th deletion variant of a prefix of length L with at most m deletions.
 // Ranks enumerate deletion sets by size (0, 1, 2, ...), each size in colex
 // order. The order is internal: records are sorted by key afterwards.
 __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
-                                               const uint32_t* __restrict__ binom /*19x19*/) {
+                                               const uint32_t* __restrict__ binom /*19x19*/,
+                                               const u128* __restrict__ hm,
+                                               const u128* __restrict__ lm) {
 	if (k == 0)
 		return p;
 	k -= 1;
 	if (k < (uint32_t)L)
-		return del_slot(p, (int)k);
+		return del_slot_t(p, (int)k, hm, lm);
 	k -= (uint32_t)L;
 	if (m >= 2) {
 		const uint32_t c2 = binom[L * 19 + 2];
@@ -137,7 +155,7 @@ __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
 			while ((b + 1) * b / 2 <= k)
 				++b;
 			const uint32_t a = k - b * (b - 1) / 2;
-			return del_slot(del_slot(p, (int)b), (int)a);
+			return del_slot_t(del_slot_t(p, (int)b, hm, lm), (int)a, hm, lm);
 		}
 		k -= c2;
 	}
@@ -151,7 +169,7 @@ __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
 				while (c + 1 < L && binom[(c + 1) * 19 + t] <= q)
 					++c;
 				q -= binom[c * 19 + t];
-				v = del_slot(v, c);  // positions come out in decreasing order
+				v = del_slot_t(v, c, hm, lm);  // positions come out in decreasing order
 			}
 			return v;
 		}
@@ -202,6 +220,7 @@ template <int MODE>
 __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	__shared__ uint64_t s_voff[GEN_TILE + 2];
 	__shared__ uint32_t s_binom[19 * 19];
+	__shared__ u128 s_hm[MAX_PREFIX], s_lm[MAX_PREFIX];
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
 	__shared__ unsigned long long s_hist[MODE == GEN_HIST ? 4096 : 1];
@@ -222,6 +241,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		}
 		s_binom[i] = c;
 	}
+	init_slot_masks(s_hm, s_lm, t, GEN_NT);
 	if (MODE == GEN_HIST) {
 		const int nb = 1 << a.part_bits;
 		for (int i = t; i < nb; i += GEN_NT)
@@ -242,6 +262,9 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 
 	const uint64_t idmask = (1ull << a.idb) - 1ull;
 	int idx = 0;
+	uint64_t kept[MODE == GEN_APPEND ? GEN_IT : 1];
+	uint32_t nkept = 0;
+#pragma unroll
 	for (int it = 0; it < GEN_IT; ++it) {
 		const int64_t r = r0 + (int64_t)it * GEN_NT + t;
 		const bool valid = r < r1;
@@ -269,7 +292,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			const int L = (int)(hiw >> 56);
 			const int m = a.distance < L ? a.distance : L;
 			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
-			const u128 v = unrank_variant(p, L, m, k, s_binom);
+			const u128 v = unrank_variant(p, L, m, k, s_binom, s_hm, s_lm);
 			rec = (hash_variant(v) & ~idmask) | ((uint64_t)sid & idmask);
 		}
 		if (MODE == GEN_DIRECT) {
@@ -280,10 +303,17 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 				atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1ull);
 		} else {
 			const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
-			const unsigned long long slot = wave_append(a.append, mine ? 1u : 0u);
-			if (mine && (int64_t)slot < a.cap)
-				a.out[slot] = rec;
+			if (mine)
+				kept[nkept++ & (GEN_IT - 1)] = rec;
 		}
+	}
+	if (MODE == GEN_APPEND) {
+		__shared__ uint32_t s_tmp[GEN_NT / WAVE + 1];
+		__shared__ unsigned long long s_base;
+		unsigned long long slot = block_append<GEN_NT>(a.append, nkept, s_tmp, &s_base);
+		for (uint32_t i = 0; i < nkept; ++i, ++slot)
+			if ((int64_t)slot < a.cap)
+				a.out[slot] = kept[i];
 	}
 	if (MODE == GEN_HIST) {
 		__syncthreads();
@@ -296,46 +326,85 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 
 // --------------------------------------------------------------- runs --
 
-// one thread per sorted record: a run is a maximal segment of equal
-// (rec >> rs); runs of >= 2 records are listed (xtn_inner.cu:34-46 does this
-// with DeviceRunLengthEncode over all keys)
-__global__ __launch_bounds__(256) void k_runs(const uint64_t* __restrict__ rec, int64_t n, int rs,
-                                              Run* __restrict__ small, int64_t small_cap,
-                                              Run* __restrict__ big, int64_t big_cap,
-                                              RunCtrs* __restrict__ ctr) {
-	const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	bool is_small = false, is_big = false;
-	uint32_t len = 0;
-	if (i < n) {
-		const uint64_t cur = rec[i] >> rs;
-		const bool head = i == 0 || (rec[i - 1] >> rs) != cur;
-		if (head && i + 1 < n && (rec[i + 1] >> rs) == cur) {
-			int64_t j = i + 2;
-			while (j < n && (rec[j] >> rs) == cur)
-				++j;
-			len = (uint32_t)(j - i);
-			is_small = len <= (uint32_t)SMALL_RUN_MAX;
-			is_big = !is_small;
+// A run is a maximal segment of sorted records with equal (rec >> rs); runs
+// of >= 2 records are listed (xtn_inner.cu:34-46 does this with
+// DeviceRunLengthEncode over all keys). Each workgroup stages a 4096-record
+// tile plus one record of halo on each side in LDS, finds the run heads of
+// the tile and appends them with one atomic per list per workgroup.
+constexpr int RN_NT = 256;
+constexpr int RN_IT = 16;
+constexpr int RN_TILE = RN_NT * RN_IT;
+
+__global__ __launch_bounds__(RN_NT) void k_runs(const uint64_t* __restrict__ rec, int64_t n, int rs,
+                                                Run* __restrict__ small, int64_t small_cap,
+                                                Run* __restrict__ big, int64_t big_cap,
+                                                RunCtrs* __restrict__ ctr) {
+	__shared__ uint64_t s_k[RN_TILE + 2];
+	__shared__ uint32_t s_tmp32[RN_NT / WAVE + 1];
+	__shared__ unsigned long long s_tmp64[RN_NT / WAVE + 1];
+	__shared__ unsigned long long s_base;
+	const int t = threadIdx.x;
+	const int64_t base = (int64_t)blockIdx.x * RN_TILE;
+	const int64_t rem = n - base;
+	const int tile_n = rem < RN_TILE ? (int)rem : RN_TILE;
+	const uint64_t NONE = ~0ull;  // shifted keys have at most 64 - rs bits, rs >= 1
+	for (int j = t; j < tile_n; j += RN_NT)
+		s_k[j + 1] = rec[base + j] >> rs;
+	if (t == 0) {
+		s_k[0] = base > 0 ? (rec[base - 1] >> rs) : NONE;
+		s_k[tile_n + 1] = base + tile_n < n ? (rec[base + tile_n] >> rs) : NONE;
+	}
+	__syncthreads();
+
+	uint32_t lens[RN_IT];
+	uint32_t ns = 0, nb = 0;
+	unsigned long long ps = 0, pb = 0;
+#pragma unroll
+	for (int it = 0; it < RN_IT; ++it) {
+		const int j = it * RN_NT + t;
+		uint32_t len = 0;
+		if (j < tile_n) {
+			const uint64_t cur = s_k[j + 1];
+			if (cur != s_k[j] && s_k[j + 2] == cur) {
+				int jj = j + 2;
+				while (jj < tile_n && s_k[jj + 1] == cur)
+					++jj;
+				int64_t g = base + jj;
+				if (jj >= tile_n)  // the run may continue past the tile
+					while (g < n && (rec[g] >> rs) == cur)
+						++g;
+				len = (uint32_t)(g - (base + j));
+			}
+		}
+		lens[it] = len;
+		const unsigned long long np = (unsigned long long)len * (len ? len - 1 : 0) / 2;
+		if (len && len <= (uint32_t)SMALL_RUN_MAX) {
+			++ns;
+			ps += np;
+		} else if (len) {
+			++nb;
+			pb += np;
 		}
 	}
-	const unsigned long long ps = wave_append(&ctr->n_small, is_small ? 1u : 0u);
-	if (is_small && (int64_t)ps < small_cap)
-		small[ps] = Run{(uint32_t)i, len};
-	const unsigned long long pb = wave_append(&ctr->n_big, is_big ? 1u : 0u);
-	if (is_big && (int64_t)pb < big_cap)
-		big[pb] = Run{(uint32_t)i, len};
-	const unsigned long long np = (unsigned long long)len * (len > 0 ? len - 1 : 0) / 2;
-	// pair bounds: small runs stay far below 2^32 per lane, big runs may not
-	unsigned long long sp = is_small ? np : 0, bp = is_big ? np : 0;
-	for (int o = WAVE / 2; o > 0; o >>= 1) {
-		sp += __shfl_down(sp, o, WAVE);
-		bp += __shfl_down(bp, o, WAVE);
-	}
-	if (lane_id() == 0) {
-		if (sp)
-			atomicAdd(&ctr->pairs_small, sp);
-		if (bp)
-			atomicAdd(&ctr->pairs_big, bp);
+	unsigned long long sb = block_append<RN_NT>(&ctr->n_small, ns, s_tmp32, &s_base);
+	unsigned long long bb = block_append<RN_NT>(&ctr->n_big, nb, s_tmp32, &s_base);
+	block_add<RN_NT>(&ctr->pairs_small, ps, s_tmp64);
+	block_add<RN_NT>(&ctr->pairs_big, pb, s_tmp64);
+#pragma unroll
+	for (int it = 0; it < RN_IT; ++it) {
+		const uint32_t len = lens[it];
+		if (!len)
+			continue;
+		const Run r{(uint32_t)(base + it * RN_NT + t), len};
+		if (len <= (uint32_t)SMALL_RUN_MAX) {
+			if ((int64_t)sb < small_cap)
+				small[sb] = r;
+			++sb;
+		} else {
+			if ((int64_t)bb < big_cap)
+				big[bb] = r;
+			++bb;
+		}
 	}
 }
 
@@ -368,7 +437,9 @@ __global__ __launch_bounds__(256) void k_pairs_small(const uint64_t* __restrict_
 			count += pair_of(ka, rec[run.start + b], idb, &p) ? 1u : 0u;
 		}
 	}
-	unsigned long long slot = wave_append(&ctr->n_cand, count);
+	__shared__ uint32_t s_tmp[256 / WAVE + 1];
+	__shared__ unsigned long long s_base;
+	unsigned long long slot = block_append<256>(&ctr->n_cand, count, s_tmp, &s_base);
 	for (uint32_t a = 0; a < run.len; ++a) {
 		const uint64_t ka = rec[run.start + a];
 		for (uint32_t b = a + 1; b < run.len; ++b) {
@@ -566,9 +637,8 @@ __global__ __launch_bounds__(256) void k_verify(const uint64_t* __restrict__ pai
 			dv = (int8_t)d;
 		}
 	}
-	unsigned long long c = (unsigned long long)__popcll(__ballot(uniq));
-	if (lane_id() == 0 && c)
-		atomicAdd(&ctr->n_unique, c);
+	__shared__ unsigned long long s_tmp[256 / WAVE + 1];
+	block_add<256>(&ctr->n_unique, uniq ? 1ull : 0ull, s_tmp);
 	if (q < n) {
 		flags[q] = keep;
 		dist[q] = dv;

@@ -234,6 +234,7 @@ int scan_excl(xtn_handle* h, const T* in, T* out, int64_t n, T* d_total) {
 	const int64_t nt = (n + SC_TILE - 1) / SC_TILE;
 	TRY(ensure(h, h->scan_tmp, sizeof(T) * (size_t)nt));
 	T* tmp = h->scan_tmp.as<T>();
+	ProfScope ps(h, "scan", 3.0 * sizeof(T) * (double)n);
 	k_scan_sums<T><<<grid_for(nt, 1), SC_NT, 0, s>>>(in, n, tmp);
 	HIP_LAUNCH_CHECK(h);
 	k_scan_single<T><<<1, SC_NT, 0, s>>>(tmp, nt, d_total);
@@ -443,7 +444,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 		k_fill_u64<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(h->ctrs.p), 8, 0ull);
 		{
 			ProfScope ps(h, "runs", (double)nrec * 8.0);
-			k_runs<<<grid_for(nrec, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), nrec, rs,
+			k_runs<<<grid_for(nrec, RN_TILE), RN_NT, 0, s>>>(h->recs.as<uint64_t>(), nrec, rs,
 			                                            h->runs_small.as<Run>(), cap_recs / 2 + 1,
 			                                            h->runs_big.as<Run>(),
 			                                            cap_recs / (SMALL_RUN_MAX + 1) + 1,

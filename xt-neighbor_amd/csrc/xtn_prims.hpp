@@ -81,6 +81,33 @@ __device__ __forceinline__ unsigned long long wave_append(unsigned long long* ct
 	return base + (inc - count);
 }
 
+// Appends `count` items per thread to a global counter with one atomic per
+// workgroup of NT threads. Returns the thread's first slot. Every thread of
+// the workgroup must call it (it contains barriers).
+template <int NT>
+__device__ __forceinline__ unsigned long long block_append(unsigned long long* ctr, uint32_t count,
+                                                           uint32_t* s_tmp,
+                                                           unsigned long long* s_base) {
+	uint32_t total;
+	const uint32_t ex = block_excl_scan<NT>(count, s_tmp, &total);
+	if (threadIdx.x == 0)
+		*s_base = total ? atomicAdd(ctr, (unsigned long long)total) : 0ull;
+	__syncthreads();
+	const unsigned long long b = *s_base;
+	__syncthreads();
+	return b + ex;
+}
+
+// adds v over the workgroup into *ctr with one atomic (all threads call it)
+template <int NT>
+__device__ __forceinline__ void block_add(unsigned long long* ctr, unsigned long long v,
+                                          unsigned long long* s_tmp) {
+	unsigned long long total;
+	block_excl_scan<NT>(v, s_tmp, &total);
+	if (threadIdx.x == 0 && total)
+		atomicAdd(ctr, total);
+}
+
 // ---------------------------------------------------------------- scan --
 
 constexpr int SC_NT = 256;
