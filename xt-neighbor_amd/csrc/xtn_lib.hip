@@ -567,11 +567,146 @@ void print_verbose(xtn_handle* h, const char* mode) {
 	       st.ms_total, st.ms_generate, st.ms_sort, st.ms_pairs, st.ms_verify, st.ms_output);
 }
 
+// Repertoire overlap (xtn.cu:281-290 overlap_mode_init, stream 4 overlap and
+// mergeOutput): exact-duplicate groups, NN search over one representative
+// per group, both overlap terms into a dense R x R matrix, one callback.
 int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, void* user) {
-	(void)info;
-	(void)callback;
-	(void)user;
-	return fail(h, XTN_ERR_STATE, "overlap mode not built yet");
+	hipStream_t s = h->stream;
+	const int32_t n = h->n;
+	xtn_stats& st = h->stats;
+	const double t0 = now_ms();
+	std::vector<int32_t> hfr((size_t)std::max(n, 1) * 2);
+	int32_t R = 0;
+	for (int32_t i = 0; i < n; ++i) {
+		if (info[i].repertoire < 0)
+			return fail(h, XTN_ERR_ARG, "negative repertoire index at row %d", i);
+		hfr[i] = info[i].frequency;
+		hfr[(size_t)n + i] = info[i].repertoire;
+		R = std::max(R, info[i].repertoire + 1);
+	}
+	const int64_t RR = (int64_t)R * R;
+	if (RR > ((int64_t)1 << 31))
+		return fail(h, XTN_ERR_CAPACITY, "%d repertoires exceed the dense overlap matrix limit (46340)", R);
+	if (n == 0) {
+		xtn_output out{nullptr, nullptr, nullptr, 0};
+		if (callback)
+			callback(user, &out);
+		return XTN_OK;
+	}
+	TRY(ensure(h, h->ov_info, sizeof(int32_t) * 2 * (size_t)n));
+	HIP_TRY(h, hipMemcpyAsync(h->ov_info.p, hfr.data(), sizeof(int32_t) * 2 * (size_t)n,
+	                          hipMemcpyHostToDevice, s));
+	const int32_t* freq_d = h->ov_info.as<int32_t>();
+	const int32_t* rep_d = freq_d + n;
+
+	// exact duplicates: hash sort, leaders, stable sort by leader position
+	const int idb = bits_for(std::max(n - 1, 1));
+	const int K = std::min(32, 64 - idb);
+	const int rs = 64 - K;
+	TRY(ensure(h, h->ov_keys, sizeof(uint64_t) * (size_t)n));
+	TRY(ensure(h, h->ov_keys_alt, sizeof(uint64_t) * (size_t)n));
+	TRY(ensure(h, h->ov_vals, sizeof(uint32_t) * (size_t)n));
+	TRY(ensure(h, h->ov_vals_alt, sizeof(uint32_t) * (size_t)n));
+	{
+		ProfScope ps(h, "ov_hash", (double)n * 28.0);
+		k_hash_rows<<<grid_for(n, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), n, idb,
+		                                              h->ov_keys.as<uint64_t>());
+		HIP_LAUNCH_CHECK(h);
+	}
+	TRY(radix_sort(h, h->ov_keys, h->ov_keys_alt, nullptr, nullptr, n, rs, 64, "ov_sort_hash"));
+	{
+		ProfScope ps(h, "ov_leaders", (double)n * 40.0);
+		k_leaders<<<grid_for(n, 256), 256, 0, s>>>(h->ov_keys.as<uint64_t>(), n, rs, idb,
+		                                            h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
+		                                            h->ov_keys_alt.as<uint64_t>(), h->ov_vals.as<uint32_t>());
+		HIP_LAUNCH_CHECK(h);
+	}
+	TRY(radix_sort(h, h->ov_keys_alt, h->ov_keys, &h->ov_vals, &h->ov_vals_alt, n, 0, bits_for(n),
+	               "ov_sort_leader"));
+	const uint64_t* lead = h->ov_keys_alt.as<uint64_t>();
+	const uint32_t* rows = h->ov_vals.as<uint32_t>();
+
+	TRY(ensure(h, h->flags, sizeof(uint32_t) * (size_t)n));
+	TRY(ensure(h, h->pos, sizeof(uint32_t) * ((size_t)n + 1)));
+	k_group_flags<<<grid_for(n, 256), 256, 0, s>>>(lead, n, h->flags.as<uint32_t>());
+	HIP_LAUNCH_CHECK(h);
+	TRY(scan_excl<uint32_t>(h, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), n, h->pos.as<uint32_t>() + n));
+	uint32_t U = 0;
+	HIP_TRY(h, hipMemcpyAsync(&U, h->pos.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_TRY(h, hipStreamSynchronize(s));
+	TRY(ensure(h, h->ov_ustart, sizeof(uint32_t) * ((size_t)U + 1)));
+	TRY(ensure(h, h->ov_umap, sizeof(uint32_t) * (size_t)U));
+	k_group_starts<<<grid_for(n, 256), 256, 0, s>>>(h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), rows, n,
+	                                                 h->ov_ustart.as<uint32_t>(), h->ov_umap.as<uint32_t>());
+	HIP_LAUNCH_CHECK(h);
+	const uint32_t nn32 = (uint32_t)n;
+	HIP_TRY(h, hipMemcpyAsync(h->ov_ustart.as<uint32_t>() + U, &nn32, sizeof(uint32_t),
+	                          hipMemcpyHostToDevice, s));
+
+	// matrix
+	TRY(ensure(h, h->ov_matrix, sizeof(unsigned long long) * (size_t)RR));
+	TRY(ensure(h, h->ov_present, sizeof(uint32_t) * (size_t)(RR + 2)));
+	k_fill_u64<<<grid_for(RR, 256), 256, 0, s>>>(h->ov_matrix.as<unsigned long long>(), RR, 0ull);
+	k_fill_u64<<<grid_for((RR + 1) / 2, 256), 256, 0, s>>>(h->ov_present.as<unsigned long long>(), (RR + 1) / 2, 0ull);
+	HIP_LAUNCH_CHECK(h);
+	OvMatrix m{h->ov_matrix.as<unsigned long long>(), h->ov_present.as<uint32_t>(), R};
+	{
+		ProfScope ps(h, "ov_exact", (double)n * 16.0);
+		k_exact_term<<<grid_for(U, 256), 256, 0, s>>>(h->ov_ustart.as<uint32_t>(), (int32_t)U, rows, freq_d,
+		                                               rep_d, m);
+		HIP_LAUNCH_CHECK(h);
+	}
+	HIP_TRY(h, hipStreamSynchronize(s));
+	const double t_dedup = now_ms() - t0;
+
+	// neighbours among the unique strings
+	TRY(nn_pipeline(h, (int32_t)U, h->ov_umap.as<uint32_t>()));
+	const int64_t npairs = h->out_len;
+	if (npairs > 0) {
+		ProfScope ps(h, "ov_neighbor", (double)npairs * 40.0);
+		k_neighbor_term<<<grid_for(npairs, 256), 256, 0, s>>>(h->out_pairs.as<int2>(), npairs,
+		                                                       h->ov_ustart.as<uint32_t>(), rows, freq_d,
+		                                                       rep_d, m);
+		HIP_LAUNCH_CHECK(h);
+	}
+
+	// present entries in (A, B) order
+	const double to = now_ms();
+	TRY(ensure(h, h->pos, sizeof(uint32_t) * ((size_t)RR + 1)));
+	TRY(scan_excl<uint32_t>(h, h->ov_present.as<uint32_t>(), h->pos.as<uint32_t>(), RR,
+	                        h->pos.as<uint32_t>() + RR));
+	uint32_t nout = 0;
+	HIP_TRY(h, hipMemcpyAsync(&nout, h->pos.as<uint32_t>() + RR, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_TRY(h, hipStreamSynchronize(s));
+	TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
+	TRY(ensure(h, h->ov_cnt, sizeof(unsigned long long) * (size_t)std::max<uint32_t>(nout, 1)));
+	k_matrix_emit<<<grid_for(RR, 256), 256, 0, s>>>(h->ov_present.as<uint32_t>(), h->pos.as<uint32_t>(),
+	                                                 h->ov_matrix.as<unsigned long long>(), RR, R,
+	                                                 h->out_pairs.as<int2>(), h->ov_cnt.as<unsigned long long>());
+	HIP_LAUNCH_CHECK(h);
+	std::vector<xtn_int2> hp(std::max<uint32_t>(nout, 1));
+	std::vector<uint64_t> hv(std::max<uint32_t>(nout, 1));
+	if (nout) {
+		HIP_TRY(h, hipMemcpyAsync(hp.data(), h->out_pairs.p, sizeof(xtn_int2) * nout, hipMemcpyDeviceToHost, s));
+		HIP_TRY(h, hipMemcpyAsync(hv.data(), h->ov_cnt.p, sizeof(uint64_t) * nout, hipMemcpyDeviceToHost, s));
+	}
+	HIP_TRY(h, hipStreamSynchronize(s));
+	collect_profile(h);
+	st.n_sequences = U;
+	st.n_output = nout;
+	st.ms_output = now_ms() - to;
+	st.ms_total = now_ms() - t0;
+	if (h->cfg.verbose)
+		printf("overlap: %d rows, %u unique, %d repertoires, dedup %.3f ms\n", n, U, R, t_dedup);
+	print_verbose(h, "overlap");
+	xtn_output out;
+	out.index_pairs = hp.data();
+	out.pairwise_distances = nullptr;
+	out.pairwise_frequencies = hv.data();
+	out.len = nout;
+	if (callback)
+		callback(user, &out);
+	return XTN_OK;
 }
 
 }  // namespace
