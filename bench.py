@@ -102,8 +102,9 @@ def main():
     st = eng.stats()
 
     step_s = (t1 - t0) / args.steps
-    kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "runs", "pairs_small", "pairs_big",
-               "sort_pairs_hist", "sort_pairs_scatter", "verify", "emit", "pack", "scan", "gen_count"]
+    kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "bucket", "runs", "pairs_small", "pairs_big",
+               "sort_seg_hist", "sort_seg_scatter", "sort_pairs_hist", "sort_pairs_scatter", "verify",
+               "emit", "pack", "scan", "gen_count"]
     prof = {k: eng.kernel_profile(k) for k in kernels}
     dom = max(prof, key=lambda k: prof[k][1])
     launches, ms, byts = prof[dom]

@@ -83,7 +83,9 @@ typedef struct {
 	                                  callback, copied on a side stream */
 	int32_t profile;               /* 1: time every kernel launch with HIP
 	                                  events (see xtn_kernel_profile) */
-	int32_t reserved;
+	int32_t bucket_capacity;       /* records one workgroup groups in LDS
+	                                  (0: 4096); larger buckets take the
+	                                  global sort path */
 } xtn_config;
 
 typedef struct xtn_handle xtn_handle;

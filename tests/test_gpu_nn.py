@@ -24,7 +24,7 @@ def engine():
 
 def run(engine, seqs=None, chars=None, offsets=None, d=1, measure=0, **kw):
     engine.configure(distance=d, measure=measure, max_records_per_chunk=kw.get("chunk", 0),
-                     output_chunk_pairs=kw.get("out_chunk", 0))
+                     output_chunk_pairs=kw.get("out_chunk", 0), bucket_capacity=kw.get("cap", 0))
     engine.set_sequences(seqs, chars, offsets)
     p, dist, ncb = engine.nn()
     return triples(p, dist), ncb
@@ -78,6 +78,16 @@ def test_runs_across_tile_boundaries(engine, copies):
         got, _ = run(engine, chars=chars, offsets=offsets, d=d)
         port, _ = oracle.nn_port(chars, offsets, d, 0)
         assert np.array_equal(got, port), d
+
+
+@pytest.mark.parametrize("n,cap", [(3000, 1000), (3000, 1430), (30000, 75)])
+def test_bucket_overflow_fallback(engine, n, cap):
+    # buckets above `cap` records take the global sort/run/pair path
+    chars, offsets = xtn.synth_nn(8, n, planted=0.3, max_edits=2)
+    base, _ = run(engine, chars=chars, offsets=offsets, d=2)
+    forced, _ = run(engine, chars=chars, offsets=offsets, d=2, cap=cap)
+    assert np.array_equal(base, forced)
+    assert np.array_equal(base, oracle.nn_port(chars, offsets, 2, 0)[0])
 
 
 @pytest.mark.parametrize("chunk", [1 << 16, 70000])

@@ -179,7 +179,7 @@ __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
 }
 
 constexpr int GEN_NT = 256;
-constexpr int GEN_IT = 8;
+constexpr int GEN_IT = 16;  // GEN_TILE == RS_TILE: a generator tile is a radix tile
 constexpr int GEN_TILE = GEN_NT * GEN_IT;
 
 enum GenMode { GEN_DIRECT = 0, GEN_HIST = 1, GEN_APPEND = 2 };
@@ -198,6 +198,10 @@ struct GenArgs {
 	int64_t cap;
 	unsigned long long* part_hist;  // GEN_HIST: 2^part_bits counters
 	unsigned long long* append;     // GEN_APPEND cursor
+	uint32_t* digit_hist;           // GEN_DIRECT, nullable: per-tile counts of the
+	int64_t ntiles;                 // 8-bit digit at digit_shift, laid out as
+	int32_t digit_shift;            // k_radix_hist writes them
+	int32_t pad2;
 };
 
 __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n, uint64_t x) {
@@ -223,7 +227,8 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	__shared__ u128 s_hm[MAX_PREFIX], s_lm[MAX_PREFIX];
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
-	__shared__ unsigned long long s_hist[MODE == GEN_HIST ? 4096 : 1];
+	__shared__ uint32_t s_hist[MODE == GEN_HIST ? 4096 : 1];
+	__shared__ uint32_t s_dig[MODE == GEN_DIRECT ? RS_BINS : 1];
 
 	const int t = threadIdx.x;
 	const int64_t r0 = (int64_t)blockIdx.x * GEN_TILE;
@@ -242,6 +247,9 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		s_binom[i] = c;
 	}
 	init_slot_masks(s_hm, s_lm, t, GEN_NT);
+	if (MODE == GEN_DIRECT)
+		for (int i = t; i < RS_BINS; i += GEN_NT)
+			s_dig[i] = 0;
 	if (MODE == GEN_HIST) {
 		const int nb = 1 << a.part_bits;
 		for (int i = t; i < nb; i += GEN_NT)
@@ -296,11 +304,14 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			rec = (hash_variant(v) & ~idmask) | ((uint64_t)sid & idmask);
 		}
 		if (MODE == GEN_DIRECT) {
-			if (valid)
+			if (valid) {
 				a.out[r] = rec;
+				if (a.digit_hist)
+					atomicAdd(&s_dig[(uint32_t)(rec >> a.digit_shift) & 0xFFu], 1u);
+			}
 		} else if (MODE == GEN_HIST) {
 			if (valid)
-				atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1ull);
+				atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1u);
 		} else {
 			const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
 			if (mine)
@@ -315,12 +326,17 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			if ((int64_t)slot < a.cap)
 				a.out[slot] = kept[i];
 	}
+	if (MODE == GEN_DIRECT && a.digit_hist) {
+		__syncthreads();
+		for (int i = t; i < RS_BINS; i += GEN_NT)
+			a.digit_hist[(int64_t)i * a.ntiles + blockIdx.x] = s_dig[i];
+	}
 	if (MODE == GEN_HIST) {
 		__syncthreads();
 		const int nb = 1 << a.part_bits;
 		for (int i = t; i < nb; i += GEN_NT)
 			if (s_hist[i])
-				atomicAdd(&a.part_hist[i], s_hist[i]);
+				atomicAdd(&a.part_hist[i], (unsigned long long)s_hist[i]);
 	}
 }
 
@@ -450,6 +466,148 @@ __global__ __launch_bounds__(256) void k_pairs_small(const uint64_t* __restrict_
 				++slot;
 			}
 		}
+	}
+}
+
+// ---------------------------------------------------------- buckets --
+//
+// After two global LSD passes the records of a key-space chunk are ordered by
+// the B = 16 hash bits below the chunk bits: every bucket (equal B bits) is a
+// contiguous segment of ~2k records. One workgroup per bucket finishes the
+// grouping in LDS: the segment is loaded, sorted by the next hash bits with
+// stable LDS radix passes, runs of equal sort bits are found and every record
+// pair of a run that shares the full stored hash and two different ids is
+// emitted (generate_pairs, kernel.cu:157-191, over all groups at once).
+// Segments longer than the LDS tile go to the global sort/run/pair kernels.
+
+// bstart[b] = first record of bucket b (records sorted by bucket bits)
+__global__ __launch_bounds__(256) void k_bucket_bounds(const uint64_t* __restrict__ rec, int64_t n,
+                                                       int bshift, uint32_t bmask, int nb,
+                                                       uint32_t* __restrict__ bstart) {
+	const int b = blockIdx.x * 256 + threadIdx.x;
+	if (b > nb)
+		return;
+	int64_t lo = 0, hi = n;
+	while (lo < hi) {
+		const int64_t mid = (lo + hi) >> 1;
+		if ((uint32_t)((rec[mid] >> bshift) & bmask) < (uint32_t)b)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	bstart[b] = (uint32_t)lo;
+}
+
+struct BucketArgs {
+	const uint64_t* rec;
+	const uint32_t* bstart;
+	int32_t nb;
+	int32_t bin_shift;  // the bucket is binned by bits [bin_shift, bin_shift + bin_bits)
+	int32_t bin_bits;   // <= BK_BIN_BITS
+	int32_t idb;
+	int32_t cap;        // largest bucket handled in LDS (<= RS_TILE)
+	int32_t ovf_cap;
+	uint64_t* pairs;
+	int64_t pcap;
+	RunCtrs* ctr;
+	uint32_t* ovf;      // overflowing bucket ids
+	unsigned long long* n_ovf;
+};
+
+constexpr int BK_BIN_BITS = 12;
+constexpr int BK_BINS = 1 << BK_BIN_BITS;
+constexpr int BK_BINS_PER_THREAD = BK_BINS / RS_NT;
+
+// One workgroup per bucket: a counting sort of the bucket's records into
+// 4096 LDS bins by 12 more hash bits (LDS atomics; the order inside a bin
+// does not matter), then every record pair inside a bin that shares the full
+// stored hash and holds two different ids is emitted.
+__global__ __launch_bounds__(RS_NT) void k_bucket(BucketArgs a) {
+	__shared__ uint64_t s_keys[RS_TILE];
+	__shared__ uint32_t s_bin[BK_BINS];
+	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+	__shared__ unsigned long long s_base;
+	const int t = threadIdx.x;
+	const int b = blockIdx.x;
+	const uint32_t s = a.bstart[b], e = a.bstart[b + 1];
+	const int cnt = (int)(e - s);
+	if (cnt < 2)
+		return;  // uniform over the workgroup
+	if (cnt > a.cap) {
+		if (t == 0) {
+			const unsigned long long slot = atomicAdd(a.n_ovf, 1ull);
+			if ((int64_t)slot < a.ovf_cap)
+				a.ovf[slot] = (uint32_t)b;
+		}
+		return;
+	}
+	const uint32_t bmask = (1u << a.bin_bits) - 1u;
+	for (int i = t; i < BK_BINS; i += RS_NT)
+		s_bin[i] = 0;
+	__syncthreads();
+	uint64_t k[RS_IT];
+	uint32_t slot_in_bin[RS_IT];
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		const int j = r * RS_NT + t;
+		if (j < cnt) {
+			k[r] = a.rec[s + j];
+			slot_in_bin[r] = atomicAdd(&s_bin[(uint32_t)(k[r] >> a.bin_shift) & bmask], 1u);
+		}
+	}
+	__syncthreads();
+	// exclusive scan over the bins, BK_BINS_PER_THREAD consecutive bins per thread
+	uint32_t c[BK_BINS_PER_THREAD];
+	uint32_t sum = 0;
+#pragma unroll
+	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
+		c[i] = s_bin[t * BK_BINS_PER_THREAD + i];
+		sum += c[i];
+	}
+	uint32_t run = block_excl_scan<RS_NT>(sum, s_scan, (uint32_t*)nullptr);
+	uint32_t count = 0;
+#pragma unroll
+	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
+		s_bin[t * BK_BINS_PER_THREAD + i] = run;
+		run += c[i];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		const int j = r * RS_NT + t;
+		if (j < cnt)
+			s_keys[s_bin[(uint32_t)(k[r] >> a.bin_shift) & bmask] + slot_in_bin[r]] = k[r];
+	}
+	__syncthreads();
+	// bins of this thread: [start, start + c[i]); s_bin holds the starts
+#pragma unroll
+	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
+		if (c[i] < 2)
+			continue;
+		const uint32_t b0 = s_bin[t * BK_BINS_PER_THREAD + i], b1 = b0 + c[i];
+		for (uint32_t x = b0; x < b1; ++x)
+			for (uint32_t y = x + 1; y < b1; ++y) {
+				uint64_t p;
+				count += pair_of(s_keys[x], s_keys[y], a.idb, &p) ? 1u : 0u;
+			}
+	}
+	unsigned long long slot = block_append<RS_NT>(&a.ctr->n_cand, count, s_scan, &s_base);
+	if (!count)
+		return;
+#pragma unroll
+	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
+		if (c[i] < 2)
+			continue;
+		const uint32_t b0 = s_bin[t * BK_BINS_PER_THREAD + i], b1 = b0 + c[i];
+		for (uint32_t x = b0; x < b1; ++x)
+			for (uint32_t y = x + 1; y < b1; ++y) {
+				uint64_t p;
+				if (pair_of(s_keys[x], s_keys[y], a.idb, &p)) {
+					if ((int64_t)slot < a.pcap)
+						a.pairs[slot] = p;
+					++slot;
+				}
+			}
 	}
 }
 

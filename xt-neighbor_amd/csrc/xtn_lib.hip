@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -75,6 +76,9 @@ struct xtn_handle {
 	std::vector<PendingEvent> pending;
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAcc> prof;
+	std::vector<PendingEvent> spans;  // stage timing, always on
+	DBuf bstart, ovf;
+	int64_t last_pairs = 0;
 };
 
 namespace {
@@ -221,6 +225,48 @@ void collect_profile(xtn_handle* h) {
 	h->pending.clear();
 }
 
+// stage spans: a pair of events per stage and chunk, summed after the run
+struct StageScope {
+	xtn_handle* h;
+	PendingEvent pe;
+	bool on = false;
+	StageScope(xtn_handle* hh, const char* stage) : h(hh) {
+		pe.name = stage;
+		pe.bytes = 0;
+		pe.start = take_event(h);
+		pe.stop = take_event(h);
+		on = pe.start && pe.stop;
+		if (on)
+			(void)hipEventRecord(pe.start, h->stream);
+	}
+	~StageScope() {
+		if (!on)
+			return;
+		(void)hipEventRecord(pe.stop, h->stream);
+		h->spans.push_back(pe);
+	}
+};
+
+void collect_stages(xtn_handle* h) {
+	for (auto& pe : h->spans) {
+		float ms = 0.f;
+		if (hipEventSynchronize(pe.stop) == hipSuccess &&
+		    hipEventElapsedTime(&ms, pe.start, pe.stop) == hipSuccess) {
+			if (pe.name == "generate")
+				h->stats.ms_generate += ms;
+			else if (pe.name == "sort")
+				h->stats.ms_sort += ms;
+			else if (pe.name == "pairs")
+				h->stats.ms_pairs += ms;
+			else if (pe.name == "verify")
+				h->stats.ms_verify += ms;
+		}
+		h->event_pool.push_back(pe.start);
+		h->event_pool.push_back(pe.stop);
+	}
+	h->spans.clear();
+}
+
 // ------------------------------------------------------------ primitives --
 
 template <typename T>
@@ -244,11 +290,12 @@ int scan_excl(xtn_handle* h, const T* in, T* out, int64_t n, T* d_total) {
 	return XTN_OK;
 }
 
-// Sorts n keys (and values when vals != nullptr) by bits [b0, b1), ping-
-// ponging between the buffer pairs; on return `keys`/`vals` name the buffers
-// holding the result.
-int radix_sort(xtn_handle* h, DBuf& keys, DBuf& keys_alt, DBuf* vals, DBuf* vals_alt, int64_t n,
-               int b0, int b1, const char* prof_name) {
+// Sorts n keys (and values when vals != nullptr) by bits [b0, b1) with one
+// stable 8-bit LSD pass per digit, ping-ponging between the pointer pairs; on
+// return `keys`/`vals` point at the buffers holding the result.
+int radix_sort_ptr(xtn_handle* h, uint64_t*& keys, uint64_t*& keys_alt, uint32_t*& vals,
+                   uint32_t*& vals_alt, int64_t n, int b0, int b1, const char* prof_name,
+                   bool first_hist_ready = false) {
 	if (n <= 1 || b0 >= b1)
 		return XTN_OK;
 	if (n > (int64_t)INT32_MAX)
@@ -257,32 +304,46 @@ int radix_sort(xtn_handle* h, DBuf& keys, DBuf& keys_alt, DBuf* vals, DBuf* vals
 	const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
 	TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * ntiles)));
 	const bool kv = vals != nullptr;
+	const std::string hist_name = std::string(prof_name) + "_hist";
+	const std::string scatter_name = std::string(prof_name) + "_scatter";
 	for (int shift = b0; shift < b1; shift += 8) {
 		const double key_bytes = (double)n * (8.0 + (kv ? 4.0 : 0.0));
-		{
-			std::string nm = std::string(prof_name) + "_hist";
-			ProfScope ps(h, nm.c_str(), key_bytes);
-			k_radix_hist<<<(unsigned)ntiles, RS_NT, 0, s>>>(keys.as<uint64_t>(), n, shift,
-			                                                 h->hist.as<uint32_t>(), ntiles);
+		if (!(first_hist_ready && shift == b0)) {
+			ProfScope ps(h, hist_name.c_str(), (double)n * 8.0);
+			k_radix_hist<<<(unsigned)ntiles, RS_NT, 0, s>>>(keys, n, shift, h->hist.as<uint32_t>(), ntiles);
 			HIP_LAUNCH_CHECK(h);
 		}
 		TRY(scan_excl<uint32_t>(h, h->hist.as<uint32_t>(), h->hist.as<uint32_t>(), RS_BINS * ntiles,
 		                        nullptr));
 		{
-			std::string nm = std::string(prof_name) + "_scatter";
-			ProfScope ps(h, nm.c_str(), 2.0 * key_bytes);
+			ProfScope ps(h, scatter_name.c_str(), 2.0 * key_bytes);
 			if (kv)
 				k_radix_scatter<true><<<(unsigned)ntiles, RS_NT, 0, s>>>(
-				    keys.as<uint64_t>(), vals->as<uint32_t>(), keys_alt.as<uint64_t>(),
-				    vals_alt->as<uint32_t>(), n, shift, h->hist.as<uint32_t>(), ntiles);
+				    keys, vals, keys_alt, vals_alt, n, shift, h->hist.as<uint32_t>(), ntiles);
 			else
 				k_radix_scatter<false><<<(unsigned)ntiles, RS_NT, 0, s>>>(
-				    keys.as<uint64_t>(), nullptr, keys_alt.as<uint64_t>(), nullptr, n, shift,
-				    h->hist.as<uint32_t>(), ntiles);
+				    keys, nullptr, keys_alt, nullptr, n, shift, h->hist.as<uint32_t>(), ntiles);
 			HIP_LAUNCH_CHECK(h);
 		}
 		std::swap(keys, keys_alt);
 		if (kv)
+			std::swap(vals, vals_alt);
+	}
+	return XTN_OK;
+}
+
+// the same over whole buffers: the DBufs are swapped so that `keys`/`vals`
+// name the result
+int radix_sort(xtn_handle* h, DBuf& keys, DBuf& keys_alt, DBuf* vals, DBuf* vals_alt, int64_t n,
+               int b0, int b1, const char* prof_name, bool first_hist_ready = false) {
+	uint64_t* k = keys.as<uint64_t>();
+	uint64_t* ka = keys_alt.as<uint64_t>();
+	uint32_t* v = vals ? vals->as<uint32_t>() : nullptr;
+	uint32_t* va = vals_alt ? vals_alt->as<uint32_t>() : nullptr;
+	TRY(radix_sort_ptr(h, k, ka, v, va, n, b0, b1, prof_name, first_hist_ready));
+	if (k != keys.as<uint64_t>()) {
+		std::swap(keys, keys_alt);
+		if (vals)
 			std::swap(*vals, *vals_alt);
 	}
 	return XTN_OK;
@@ -308,6 +369,83 @@ double now_ms() {
 
 // ----------------------------------------------------------- NN pipeline --
 
+constexpr int64_t BUCKET_TARGET = 2800;  // mean records per LDS bucket
+constexpr int KEY_BITS = 32;             // hash bits that order records into runs
+
+inline unsigned long long* ctr_field(xtn_handle* h, size_t offset) {
+	return reinterpret_cast<unsigned long long*>(static_cast<char*>(h->ctrs.p) + offset);
+}
+
+int set_ctr(xtn_handle* h, size_t offset, unsigned long long v) {
+	k_fill_u64<<<1, 256, 0, h->stream>>>(ctr_field(h, offset), 1, v);
+	HIP_LAUNCH_CHECK(h);
+	return XTN_OK;
+}
+
+// Global path for one bucket that does not fit the LDS tile: sort the
+// segment by the run bits, list its runs, emit the pairs of every run.
+int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, int sort_hi, int idb,
+                    int64_t* n_pairs) {
+	hipStream_t s = h->stream;
+	uint64_t* seg = h->recs.as<uint64_t>() + seg_start;
+	uint64_t* k = seg;
+	uint64_t* ka = h->recs_alt.as<uint64_t>() + seg_start;
+	uint32_t* nv = nullptr;
+	uint32_t* nva = nullptr;
+	TRY(radix_sort_ptr(h, k, ka, nv, nva, seg_len, rs, sort_hi, "sort_seg"));
+	if (k != seg)
+		HIP_TRY(h, hipMemcpyAsync(seg, k, sizeof(uint64_t) * (size_t)seg_len, hipMemcpyDeviceToDevice, s));
+	k_fill_u64<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(h->ctrs.p), 4, 0ull);
+	TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)*n_pairs));
+	const int64_t small_cap = seg_len / 2 + 1, big_cap = seg_len / (SMALL_RUN_MAX + 1) + 1;
+	{
+		ProfScope ps(h, "runs", (double)seg_len * 8.0);
+		k_runs<<<grid_for(seg_len, RN_TILE), RN_NT, 0, s>>>(seg, seg_len, rs, h->runs_small.as<Run>(), small_cap,
+		                                                   h->runs_big.as<Run>(), big_cap, h->ctrs.as<RunCtrs>());
+		HIP_LAUNCH_CHECK(h);
+	}
+	RunCtrs rc{};
+	TRY(read_ctrs(h, &rc));
+	if ((int64_t)rc.n_small > small_cap || (int64_t)rc.n_big > big_cap)
+		return fail(h, XTN_ERR_HIP, "run list overflow (%llu small, %llu big)",
+		            (unsigned long long)rc.n_small, (unsigned long long)rc.n_big);
+	const uint64_t bound = rc.pairs_small + rc.pairs_big;
+	if ((uint64_t)*n_pairs + bound > (uint64_t)INT32_MAX)
+		return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31");
+	const int64_t need = *n_pairs + (int64_t)bound;
+	if ((size_t)need * sizeof(uint64_t) > h->pairs.cap)
+		TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)need, sizeof(uint64_t) * (size_t)*n_pairs));
+	const int64_t pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
+	if (rc.n_small) {
+		ProfScope ps(h, "pairs_small", (double)rc.n_small * 8.0);
+		k_pairs_small<<<grid_for((int64_t)rc.n_small, 256), 256, 0, s>>>(
+		    seg, h->runs_small.as<Run>(), (int64_t)rc.n_small, idb, h->pairs.as<uint64_t>(), pcap,
+		    h->ctrs.as<RunCtrs>());
+		HIP_LAUNCH_CHECK(h);
+	}
+	if (rc.n_big) {
+		TRY(ensure(h, h->big_pre, sizeof(uint64_t) * ((size_t)rc.n_big + 1)));
+		k_big_counts<<<grid_for((int64_t)rc.n_big, 256), 256, 0, s>>>(h->runs_big.as<Run>(), (int64_t)rc.n_big,
+		                                                              h->big_pre.as<uint64_t>());
+		HIP_LAUNCH_CHECK(h);
+		TRY(scan_excl<uint64_t>(h, h->big_pre.as<uint64_t>(), h->big_pre.as<uint64_t>(), (int64_t)rc.n_big,
+		                        h->big_pre.as<uint64_t>() + rc.n_big));
+		const unsigned g = (unsigned)std::min<uint64_t>((rc.pairs_big + 255) / 256, 16384);
+		ProfScope ps(h, "pairs_big", (double)rc.pairs_big * 16.0);
+		k_pairs_big<<<g > 0 ? g : 1, 256, 0, s>>>(seg, h->runs_big.as<Run>(), h->big_pre.as<uint64_t>(),
+		                                          (int64_t)rc.n_big, idb, h->pairs.as<uint64_t>(), pcap,
+		                                          h->ctrs.as<RunCtrs>());
+		HIP_LAUNCH_CHECK(h);
+	}
+	TRY(read_ctrs(h, &rc));
+	if ((int64_t)rc.n_cand > pcap)
+		return fail(h, XTN_ERR_HIP, "pair buffer overflow (%llu > %lld)", (unsigned long long)rc.n_cand,
+		            (long long)pcap);
+	h->stats.n_runs += (int64_t)(rc.n_small + rc.n_big);
+	*n_pairs = (int64_t)rc.n_cand;
+	return XTN_OK;
+}
+
 // Runs the candidate search and verification over `nseq` ids. seq_map (device,
 // nullable) maps an id to its input row. Leaves sorted (x, y, distance)
 // triples in out_pairs/out_dist and their count in h->out_len.
@@ -332,8 +470,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 	TRY(ensure(h, h->pk, sizeof(uint64_t) * 2 * (size_t)nseq));
 	TRY(ensure(h, h->voff, sizeof(uint64_t) * ((size_t)nseq + 1)));
 	TRY(ensure(h, h->ctrs, sizeof(RunCtrs)));
-	TRY(ensure(h, h->scalar, 64));
 	{
+		StageScope sg(h, "generate");
 		ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
 		k_pack<<<grid_for(nseq, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
 		                                            seq_map, nseq, d, h->pk.as<uint64_t>(),
@@ -347,30 +485,31 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 	                          hipMemcpyDeviceToHost, s));
 	HIP_TRY(h, hipStreamSynchronize(s));
 	st.n_variants = (int64_t)V;
-	const double t_gen0 = now_ms();
 
-	// chunk plan: records of one chunk (plus sort buffer, run lists and
-	// histograms, ~32 B per record) must fit the budget
+	// chunk plan: the records of one chunk (plus the sort buffer, ~32 B per
+	// record in all) must fit the HBM budget, and its buckets must stay near
+	// BUCKET_TARGET records
 	int64_t budget = h->cfg.max_records_per_chunk;
 	if (budget <= 0) {
 		size_t fr = 0, tot = 0;
 		HIP_TRY(h, hipMemGetInfo(&fr, &tot));
 		budget = (int64_t)((double)fr * 0.6 / 32.0);
-		budget = std::min<int64_t>(budget, (int64_t)1 << 31);
+		budget = std::min<int64_t>(budget, (int64_t)65536 * BUCKET_TARGET);
 	}
 	budget = std::max<int64_t>(budget, 1 << 16);
 	int pbits = 0;
 	while (((int64_t)V >> pbits) > budget && pbits < 12)
 		++pbits;
 	// partitions are hash buckets: give them 25% headroom over the mean
-	while (pbits < 12 && ((int64_t)V >> pbits) + ((int64_t)V >> (pbits + 2)) > budget && pbits > 0)
+	while (pbits > 0 && pbits < 12 && ((int64_t)V >> pbits) + ((int64_t)V >> (pbits + 2)) > budget)
 		++pbits;
 	const int nparts = 1 << pbits;
 	const int hash_bits = 64 - idb;
-	const int K = std::min(32, hash_bits - pbits);
-	if (K < 8)
+	const int K = std::min(KEY_BITS, hash_bits - pbits);
+	if (K < 16)
 		return fail(h, XTN_ERR_CAPACITY, "too few key bits (%d sequences, %d chunks)", nseq, nparts);
-	const int rs = 64 - pbits - K;
+	const int hi = 64 - pbits;
+	const int rs = hi - K;
 	st.n_chunks = nparts;
 
 	GenArgs ga{};
@@ -390,6 +529,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 		k_fill_u64<<<grid_for(nparts, 256), 256, 0, s>>>(h->part_hist.as<unsigned long long>(), nparts, 0ull);
 		ga.part_hist = h->part_hist.as<unsigned long long>();
 		{
+			StageScope sg(h, "generate");
 			ProfScope ps(h, "gen_count", (double)nseq * 16.0);
 			k_gen<GEN_HIST><<<gen_grid, GEN_NT, 0, s>>>(ga);
 			HIP_LAUNCH_CHECK(h);
@@ -403,114 +543,126 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 	} else {
 		part_count[0] = V;
 	}
-	if (cap_recs > (int64_t)UINT32_MAX - 1)
-		return fail(h, XTN_ERR_CAPACITY, "chunk of %lld records exceeds 2^32", (long long)cap_recs);
+	if (cap_recs > (int64_t)INT32_MAX)
+		return fail(h, XTN_ERR_CAPACITY, "chunk of %lld records exceeds 2^31", (long long)cap_recs);
 	TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)cap_recs));
 	TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)cap_recs));
 	TRY(ensure(h, h->runs_small, sizeof(Run) * (size_t)(cap_recs / 2 + 1)));
 	TRY(ensure(h, h->runs_big, sizeof(Run) * (size_t)(cap_recs / (SMALL_RUN_MAX + 1) + 1)));
+	TRY(ensure(h, h->bstart, sizeof(uint32_t) * (65536 + 1)));
+	TRY(ensure(h, h->ovf, sizeof(uint32_t) * 65536));
+	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, RS_TILE) : RS_TILE;
+
+	// candidate pair buffer, sized from the last run; regrown on overflow
+	const int64_t pguess = std::max<int64_t>({h->last_pairs + h->last_pairs / 4, (int64_t)V / 16, 1 << 16});
+	if ((size_t)pguess * sizeof(uint64_t) > h->pairs.cap)
+		TRY(ensure(h, h->pairs, sizeof(uint64_t) * (size_t)pguess));
 
 	int64_t n_pairs = 0;
-	double ms_gen = 0, ms_sort = 0, ms_pairs = 0;
 	for (int part = 0; part < nparts; ++part) {
 		const int64_t nrec = (int64_t)part_count[part];
 		if (nrec < 2)
 			continue;
-		double ta = now_ms();
 		ga.out = h->recs.as<uint64_t>();
 		ga.cap = cap_recs;
-		if (nparts == 1) {
-			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0);
-			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
-			HIP_LAUNCH_CHECK(h);
+		// bucket bits: the B hash bits right below the chunk bits
+		int B = nrec <= BUCKET_TARGET ? 0 : (nrec <= 256 * BUCKET_TARGET ? 8 : 16);
+		B = std::min(B, K);
+		// the single-chunk generator also counts the first sort digit per tile
+		const bool fused_hist = nparts == 1 && B > 0;
+		const int64_t rtiles = (nrec + RS_TILE - 1) / RS_TILE;
+		if (fused_hist) {
+			TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
+			ga.digit_hist = h->hist.as<uint32_t>();
+			ga.ntiles = rtiles;
+			ga.digit_shift = hi - B;
 		} else {
-			ga.part_id = (uint32_t)part;
-			ga.append = &h->ctrs.as<RunCtrs>()->append;
-			k_fill_u64<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(h->ctrs.p), 8, 0ull);
-			ProfScope ps(h, "gen", (double)nrec * 8.0 + (double)nseq * 24.0);
-			k_gen<GEN_APPEND><<<gen_grid, GEN_NT, 0, s>>>(ga);
-			HIP_LAUNCH_CHECK(h);
+			ga.digit_hist = nullptr;
 		}
-		HIP_TRY(h, hipStreamSynchronize(s));
-		double tb = now_ms();
-		ms_gen += tb - ta;
-
-		TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, rs, rs + K, "sort_rec"));
-		HIP_TRY(h, hipStreamSynchronize(s));
-		double tc = now_ms();
-		ms_sort += tc - tb;
-
-		// runs
-		k_fill_u64<<<1, 256, 0, s>>>(reinterpret_cast<unsigned long long*>(h->ctrs.p), 8, 0ull);
 		{
-			ProfScope ps(h, "runs", (double)nrec * 8.0);
-			k_runs<<<grid_for(nrec, RN_TILE), RN_NT, 0, s>>>(h->recs.as<uint64_t>(), nrec, rs,
-			                                            h->runs_small.as<Run>(), cap_recs / 2 + 1,
-			                                            h->runs_big.as<Run>(),
-			                                            cap_recs / (SMALL_RUN_MAX + 1) + 1,
-			                                            h->ctrs.as<RunCtrs>());
-			HIP_LAUNCH_CHECK(h);
+			StageScope sg(h, "generate");
+			if (nparts == 1) {
+				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0);
+				k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
+				HIP_LAUNCH_CHECK(h);
+			} else {
+				ga.part_id = (uint32_t)part;
+				ga.append = ctr_field(h, offsetof(RunCtrs, append));
+				TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
+				ProfScope ps(h, "gen", (double)nrec * 8.0 + (double)nseq * 24.0);
+				k_gen<GEN_APPEND><<<gen_grid, GEN_NT, 0, s>>>(ga);
+				HIP_LAUNCH_CHECK(h);
+			}
 		}
+		if (B > 0) {
+			StageScope sg(h, "sort");
+			TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, hi - B, hi, "sort_rec",
+			               fused_hist));
+		}
+		const int nb = 1 << B;
+		BucketArgs ba{};
+		ba.rec = h->recs.as<uint64_t>();
+		ba.bstart = h->bstart.as<uint32_t>();
+		ba.nb = nb;
+		// bins: the (up to) 12 stored hash bits right below the bucket bits
+		ba.bin_bits = std::min(BK_BIN_BITS, hi - B - idb);
+		ba.bin_shift = hi - B - ba.bin_bits;
+		ba.idb = idb;
+		ba.cap = bucket_cap;
+		ba.ovf_cap = 65536;
+		ba.ctr = h->ctrs.as<RunCtrs>();
+		ba.ovf = h->ovf.as<uint32_t>();
+		ba.n_ovf = ctr_field(h, offsetof(RunCtrs, append));
 		RunCtrs rc{};
-		TRY(read_ctrs(h, &rc));
-		if ((int64_t)rc.n_small > cap_recs / 2 + 1 || (int64_t)rc.n_big > cap_recs / (SMALL_RUN_MAX + 1) + 1)
-			return fail(h, XTN_ERR_HIP, "run list overflow (%llu small, %llu big)",
-			            (unsigned long long)rc.n_small, (unsigned long long)rc.n_big);
-		st.n_runs += (int64_t)(rc.n_small + rc.n_big);
-		const uint64_t bound = rc.pairs_small + rc.pairs_big;
-		if ((uint64_t)n_pairs + bound > (uint64_t)INT32_MAX)
-			return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31 (%llu)",
-			            (unsigned long long)((uint64_t)n_pairs + bound));
-		const int64_t need = n_pairs + (int64_t)bound;
-		if ((size_t)need * sizeof(uint64_t) > h->pairs.cap)
-			TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)std::max<int64_t>(need, 2 * n_pairs),
+		for (int attempt = 0; attempt < 2; ++attempt) {
+			StageScope sg(h, "pairs");
+			k_bucket_bounds<<<grid_for(nb + 1, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), nrec, B > 0 ? hi - B : 0,
+			                                                       (uint32_t)(nb - 1), nb, h->bstart.as<uint32_t>());
+			HIP_LAUNCH_CHECK(h);
+			TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
+			TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)n_pairs));
+			ba.pairs = h->pairs.as<uint64_t>();
+			ba.pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
+			{
+				ProfScope ps(h, "bucket", (double)nrec * 8.0);
+				k_bucket<<<(unsigned)nb, RS_NT, 0, s>>>(ba);
+				HIP_LAUNCH_CHECK(h);
+			}
+			TRY(read_ctrs(h, &rc));
+			if ((int64_t)rc.n_cand <= ba.pcap)
+				break;
+			if (attempt == 1)
+				return fail(h, XTN_ERR_HIP, "pair buffer overflow after regrow");
+			if (rc.n_cand > (unsigned long long)INT32_MAX)
+				return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31 (%llu)", (unsigned long long)rc.n_cand);
+			TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)(rc.n_cand + rc.n_cand / 4),
 			              sizeof(uint64_t) * (size_t)n_pairs));
-		// append cursor starts after the pairs of earlier chunks
-		k_fill_u64<<<1, 256, 0, s>>>(&h->ctrs.as<RunCtrs>()->n_cand, 1, (unsigned long long)n_pairs);
-		const int64_t pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
-		if (rc.n_small) {
-			ProfScope ps(h, "pairs_small", (double)rc.n_small * 8.0);
-			k_pairs_small<<<grid_for((int64_t)rc.n_small, 256), 256, 0, s>>>(
-			    h->recs.as<uint64_t>(), h->runs_small.as<Run>(), (int64_t)rc.n_small, idb,
-			    h->pairs.as<uint64_t>(), pcap, h->ctrs.as<RunCtrs>());
-			HIP_LAUNCH_CHECK(h);
 		}
-		if (rc.n_big) {
-			TRY(ensure(h, h->big_pre, sizeof(uint64_t) * ((size_t)rc.n_big + 1)));
-			k_big_counts<<<grid_for((int64_t)rc.n_big, 256), 256, 0, s>>>(
-			    h->runs_big.as<Run>(), (int64_t)rc.n_big, h->big_pre.as<uint64_t>());
-			HIP_LAUNCH_CHECK(h);
-			TRY(scan_excl<uint64_t>(h, h->big_pre.as<uint64_t>(), h->big_pre.as<uint64_t>(),
-			                        (int64_t)rc.n_big, h->big_pre.as<uint64_t>() + rc.n_big));
-			const uint64_t tb_pairs = rc.pairs_big;
-			const unsigned g = (unsigned)std::min<uint64_t>((tb_pairs + 255) / 256, 16384);
-			ProfScope ps(h, "pairs_big", (double)tb_pairs * 16.0);
-			k_pairs_big<<<g > 0 ? g : 1, 256, 0, s>>>(h->recs.as<uint64_t>(), h->runs_big.as<Run>(),
-			                                          h->big_pre.as<uint64_t>(), (int64_t)rc.n_big, idb,
-			                                          h->pairs.as<uint64_t>(), pcap, h->ctrs.as<RunCtrs>());
-			HIP_LAUNCH_CHECK(h);
-		}
-		TRY(read_ctrs(h, &rc));
-		if ((int64_t)rc.n_cand > pcap)
-			return fail(h, XTN_ERR_HIP, "pair buffer overflow (%llu > %lld)",
-			            (unsigned long long)rc.n_cand, (long long)pcap);
 		n_pairs = (int64_t)rc.n_cand;
-		ms_pairs += now_ms() - tc;
+		const int64_t n_ovf = (int64_t)rc.append;
+		if (n_ovf > 0) {
+			StageScope sg(h, "pairs");
+			std::vector<uint32_t> ids((size_t)n_ovf), bs((size_t)nb + 1);
+			HIP_TRY(h, hipMemcpyAsync(ids.data(), h->ovf.p, sizeof(uint32_t) * (size_t)n_ovf, hipMemcpyDeviceToHost, s));
+			HIP_TRY(h, hipMemcpyAsync(bs.data(), h->bstart.p, sizeof(uint32_t) * ((size_t)nb + 1),
+			                          hipMemcpyDeviceToHost, s));
+			HIP_TRY(h, hipStreamSynchronize(s));
+			for (uint32_t b : ids)
+				TRY(bucket_fallback(h, bs[b], (int64_t)bs[b + 1] - bs[b], rs, hi - B, idb, &n_pairs));
+		}
 	}
 	st.n_candidates = n_pairs;
-	st.ms_generate = ms_gen;
-	st.ms_sort = ms_sort;
-	st.ms_pairs = ms_pairs;
+	h->last_pairs = n_pairs;
 
 	// sort + unique + verify
-	const double tv = now_ms();
 	if (n_pairs > 0) {
+		StageScope sg(h, "verify");
 		TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)n_pairs));
 		TRY(radix_sort(h, h->pairs, h->pairs_alt, nullptr, nullptr, n_pairs, 0, 2 * idb, "sort_pairs"));
 		TRY(ensure(h, h->flags, sizeof(uint32_t) * (size_t)n_pairs));
 		TRY(ensure(h, h->pos, sizeof(uint32_t) * ((size_t)n_pairs + 1)));
 		TRY(ensure(h, h->dist, (size_t)n_pairs));
-		k_fill_u64<<<1, 256, 0, s>>>(&h->ctrs.as<RunCtrs>()->n_unique, 1, 0ull);
+		TRY(set_ctr(h, offsetof(RunCtrs, n_unique), 0ull));
 		{
 			ProfScope ps(h, "verify", (double)n_pairs * 13.0);
 			k_verify<<<grid_for(n_pairs, 256), 256, 0, s>>>(
@@ -520,7 +672,9 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 		}
 		TRY(scan_excl<uint32_t>(h, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), n_pairs,
 		                        h->pos.as<uint32_t>() + n_pairs));
-		uint32_t nout = 0;
+	}
+	uint32_t nout = 0;
+	if (n_pairs > 0) {
 		HIP_TRY(h, hipMemcpyAsync(&nout, h->pos.as<uint32_t>() + n_pairs, sizeof(uint32_t),
 		                          hipMemcpyDeviceToHost, s));
 		RunCtrs rc{};
@@ -529,21 +683,19 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 		TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
 		TRY(ensure(h, h->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
 		{
+			StageScope sg(h, "verify");
 			ProfScope ps(h, "emit", (double)n_pairs * 9.0 + (double)nout * 9.0);
 			k_emit_nn<<<grid_for(n_pairs, 256), 256, 0, s>>>(
 			    h->pairs.as<uint64_t>(), n_pairs, idb, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(),
 			    h->dist.as<int8_t>(), h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
 			HIP_LAUNCH_CHECK(h);
 		}
-		HIP_TRY(h, hipStreamSynchronize(s));
-		h->out_len = nout;
-	} else {
-		HIP_TRY(h, hipStreamSynchronize(s));
 	}
+	HIP_TRY(h, hipStreamSynchronize(s));
+	h->out_len = nout;
 	st.n_output = h->out_len;
-	st.ms_verify = now_ms() - tv;
 	st.ms_total = now_ms() - t0;
-	(void)t_gen0;
+	collect_stages(h);
 	collect_profile(h);
 	return XTN_OK;
 }

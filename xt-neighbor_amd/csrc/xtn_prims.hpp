@@ -234,41 +234,36 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid) {
 	return peers;
 }
 
-template <bool KV>
-__global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
-                                                         const uint32_t* __restrict__ vals_in,
-                                                         uint64_t* __restrict__ keys_out,
-                                                         uint32_t* __restrict__ vals_out, int64_t n,
-                                                         int shift, const uint32_t* __restrict__ offs,
-                                                         int64_t ntiles) {
-	__shared__ uint64_t s_keys[RS_TILE];
-	__shared__ uint32_t s_vals[KV ? RS_TILE : 1];
-	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
-	__shared__ uint32_t s_dpre[RS_BINS];
-	__shared__ uint32_t s_gbase[RS_BINS];
-	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+// Tiles are handed out so that each of the 8 XCDs (workgroups are dispatched
+// to XCDs round-robin) owns one contiguous range of tiles: consecutive tiles
+// then write their digit runs into the same XCD's L2, where the partial lines
+// of neighbouring tiles merge before write-back.
+__device__ __forceinline__ int64_t xcd_tile(int64_t i, int64_t ntiles) {
+	constexpr int64_t NX = 8;
+	const int64_t xcd = i % NX, local = i / NX;
+	const int64_t q = ntiles / NX, r = ntiles % NX;
+	return xcd * q + (xcd < r ? xcd : r) + local;
+}
 
+// Stable rank of a tile of up to RS_TILE keys by the 8-bit digit at `shift`.
+// Wave w holds the keys of tile positions [w*RS_SEG, (w+1)*RS_SEG), round r
+// lane l holding position w*RS_SEG + r*64 + l. On return lp[r] is the key's
+// position in the tile sorted by digit (stable), or 0xFFFFFFFF when invalid,
+// and s_dpre[d] the first position of digit d. Every thread must call it.
+__device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_valid, int shift,
+                                          uint32_t (&lp)[RS_IT], uint32_t (*s_cnt)[RS_BINS],
+                                          uint32_t* s_dpre, uint32_t* s_scan) {
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
 	for (int i = t; i < RS_WAVES * RS_BINS; i += RS_NT)
 		(&s_cnt[0][0])[i] = 0;
 	__syncthreads();
-
-	const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-	const int64_t seg = base + (int64_t)w * RS_SEG;
 	const uint64_t lt = lanemask_lt();
-	uint64_t k[RS_IT];
-	uint32_t v[RS_IT];
 	uint32_t rk[RS_IT];
-
 #pragma unroll
 	for (int r = 0; r < RS_IT; ++r) {
-		const int64_t idx = seg + (int64_t)r * WAVE + lane;
-		const bool valid = idx < n;
-		k[r] = valid ? keys_in[idx] : 0ull;
-		if (KV)
-			v[r] = valid ? vals_in[idx] : 0u;
+		const bool valid = r * WAVE + lane < w_valid;
 		const uint32_t d = (uint32_t)(k[r] >> shift) & 0xFFu;
 		const uint64_t peers = match_digit8(d, valid);
 		const uint32_t before = s_cnt[w][d];
@@ -278,7 +273,6 @@ __global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restr
 		rk[r] = valid ? ((rank << 8) | d) : 0xFFFFFFFFu;
 	}
 	__syncthreads();
-
 	{
 		const int d = t;
 		uint32_t c[RS_WAVES];
@@ -296,18 +290,54 @@ __global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restr
 			s_cnt[ww][d] = run;
 			run += c[ww];
 		}
-		s_gbase[d] = offs[(int64_t)d * ntiles + blockIdx.x];
 	}
 	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r)
+		lp[r] = rk[r] == 0xFFFFFFFFu ? 0xFFFFFFFFu : s_cnt[w][rk[r] & 0xFFu] + (rk[r] >> 8);
+}
 
+template <bool KV>
+__global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
+                                                         const uint32_t* __restrict__ vals_in,
+                                                         uint64_t* __restrict__ keys_out,
+                                                         uint32_t* __restrict__ vals_out, int64_t n,
+                                                         int shift, const uint32_t* __restrict__ offs,
+                                                         int64_t ntiles) {
+	__shared__ uint64_t s_keys[RS_TILE];
+	__shared__ uint32_t s_vals[KV ? RS_TILE : 1];
+	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+	__shared__ uint32_t s_dpre[RS_BINS];
+	__shared__ uint32_t s_gbase[RS_BINS];
+	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+
+	const int t = threadIdx.x;
+	const int lane = lane_id();
+	const int w = t / WAVE;
+	const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+	const int64_t base = tile * RS_TILE;
+	const int64_t seg = base + (int64_t)w * RS_SEG;
+	const int64_t wrem = n - seg;
+	const int w_valid = wrem <= 0 ? 0 : (wrem < RS_SEG ? (int)wrem : RS_SEG);
+	uint64_t k[RS_IT];
+	uint32_t v[RS_IT];
 #pragma unroll
 	for (int r = 0; r < RS_IT; ++r) {
-		if (rk[r] != 0xFFFFFFFFu) {
-			const uint32_t d = rk[r] & 0xFFu;
-			const uint32_t lp = s_cnt[w][d] + (rk[r] >> 8);
-			s_keys[lp] = k[r];
+		const bool valid = r * WAVE + lane < w_valid;
+		const int64_t idx = seg + (int64_t)r * WAVE + lane;
+		k[r] = valid ? keys_in[idx] : 0ull;
+		if (KV)
+			v[r] = valid ? vals_in[idx] : 0u;
+	}
+	s_gbase[t] = offs[(int64_t)t * ntiles + tile];
+	uint32_t lp[RS_IT];
+	rank_tile(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		if (lp[r] != 0xFFFFFFFFu) {
+			s_keys[lp[r]] = k[r];
 			if (KV)
-				s_vals[lp] = v[r];
+				s_vals[lp[r]] = v[r];
 		}
 	}
 	__syncthreads();
@@ -324,6 +354,30 @@ __global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restr
 				vals_out[g] = s_vals[i];
 		}
 	}
+}
+
+// One stable LSD pass over the n <= RS_TILE keys held in s_keys, in place.
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void lds_radix_pass(uint64_t* s_keys, int n, int shift,
+                                               uint32_t (*s_cnt)[RS_BINS], uint32_t* s_dpre,
+                                               uint32_t* s_scan) {
+	const int lane = lane_id();
+	const int w = threadIdx.x / WAVE;
+	const int wrem = n - w * RS_SEG;
+	const int w_valid = wrem <= 0 ? 0 : (wrem < RS_SEG ? wrem : RS_SEG);
+	uint64_t k[RS_IT];
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		const int idx = w * RS_SEG + r * WAVE + lane;
+		k[r] = r * WAVE + lane < w_valid ? s_keys[idx] : 0ull;
+	}
+	uint32_t lp[RS_IT];
+	rank_tile(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r)
+		if (lp[r] != 0xFFFFFFFFu)
+			s_keys[lp[r]] = k[r];
+	__syncthreads();
 }
 
 }  // namespace xtn

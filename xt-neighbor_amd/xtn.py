@@ -66,7 +66,7 @@ class Config(ctypes.Structure):
                 ("verbose", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("max_records_per_chunk", ctypes.c_int64),
                 ("output_chunk_pairs", ctypes.c_int64),
-                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("profile", ctypes.c_int32), ("bucket_capacity", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -231,7 +231,7 @@ class Engine:
 
     def __init__(self, distance: int = 1, measure: int | str = LEVENSHTEIN, device: int = 0,
                  max_records_per_chunk: int = 0, output_chunk_pairs: int = 0,
-                 profile: bool = False, verbose: bool = False):
+                 profile: bool = False, verbose: bool = False, bucket_capacity: int = 0):
         self.lib = load_library()
         self.cfg = Config()
         self.lib.xtn_config_init(ctypes.byref(self.cfg))
@@ -242,6 +242,7 @@ class Engine:
         self.cfg.output_chunk_pairs = output_chunk_pairs
         self.cfg.profile = 1 if profile else 0
         self.cfg.verbose = 1 if verbose else 0
+        self.cfg.bucket_capacity = bucket_capacity
         self.h = ctypes.c_void_p()
         st = self.lib.xtn_open(ctypes.byref(self.h), ctypes.byref(self.cfg))
         if st != 0:
