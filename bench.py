@@ -67,6 +67,83 @@ def cpu_baseline(chars, offsets, d, measure, sample):
                       f"{dt:.1f} s, {st['n_output']} output pairs"}
 
 
+def batched_vs_plain(eng, offsets, n_sets, seed):
+    """the same rows as n_sets query sets of log-uniform size: one pipeline
+    pass for all sets (batched) against one pass per set (plain)"""
+    n = len(offsets) - 1
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(100), np.log(10000), n_sets))
+    cuts = np.concatenate([[0], np.cumsum(sizes)])
+    so = np.round(cuts / cuts[-1] * n).astype(np.uint32)
+    eng.kernel_profile_reset()
+    eng.configure(profile=False)
+    out = {}
+    for name, batched in (("plain", False), ("batched", True)):
+        eng.nn_sets(so, batched=batched, collect=False)  # warm-up
+        hip_sync()
+        t0 = time.perf_counter()
+        eng.nn_sets(so, batched=batched, collect=False)
+        hip_sync()
+        dt = time.perf_counter() - t0
+        out[f"{name}_ms"] = dt * 1e3
+        out[f"{name}_seqs_per_s"] = n / dt
+        out[f"{name}_output_pairs"] = eng.stats()["n_output"]
+    out["sets"] = n_sets
+    out["rows"] = n
+    out["speedup"] = out["plain_ms"] / out["batched_ms"]
+    return out
+
+
+def served(n_index, n_queries, rate_hz, seed):
+    """query sets (log-uniform 1k..100k rows, 30% planted relatives of index
+    rows) arriving as a Poisson process against a resident 10M-row index,
+    Levenshtein d=1; latency is submit to completion"""
+    import threading
+
+    ichars, ioffs = xtn.synth_nn(seed, n_index, 15.0, 2.0, 8, 22, 0.25, 1)
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(100000), n_queries)).astype(int)
+    queries = [xtn.synth_queries(seed * 1000 + k, ichars, ioffs, int(sz), planted=0.3)
+               for k, sz in enumerate(sizes)]
+    t_build = time.perf_counter()
+    srv = xtn.Server(ichars, ioffs, distance=1, slots=4, max_query_rows=1 << 17)
+    build_ms = (time.perf_counter() - t_build) * 1e3
+    # warm-up: one query through every slot
+    for t in [srv.submit(*queries[k % n_queries]) for k in range(4)]:
+        srv.wait(t)
+    gaps = rng.exponential(1.0 / rate_hz, n_queries)
+    tickets = []
+    lat = [0.0] * n_queries
+    pairs = [0] * n_queries
+    done_at = [0.0] * n_queries
+
+    def waiter():
+        for k in range(n_queries):
+            while len(tickets) <= k:
+                time.sleep(0.0002)
+            p, _, l = srv.wait(tickets[k])
+            lat[k], pairs[k], done_at[k] = l, len(p), time.perf_counter()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    next_t = t0
+    for k in range(n_queries):
+        next_t += gaps[k]
+        delay = next_t - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(srv.submit(*queries[k]))
+    th.join()
+    wall = max(done_at) - t0
+    srv.close()
+    nq = int(sizes.sum())
+    return {"index_seqs": n_index, "queries": n_queries, "query_seqs": nq, "arrival_rate_hz": rate_hz,
+            "slots": 4, "index_build_ms": build_ms, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "seqs_per_s": nq / wall, "cross_pairs": int(sum(pairs)),
+            "wall_s": wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +152,23 @@ def main():
     ap.add_argument("--workload", default="nn-1M-cdr3-lev-d2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=300_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-sets", type=int, default=1000)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-index", type=int, default=10_000_000)
+    ap.add_argument("--served-queries", type=int, default=64)
+    ap.add_argument("--served-rate", type=float, default=200.0)
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.probe_copy:
+        probe = ctypes.CDLL(os.path.join(REPO, "xt-neighbor_amd", "lib", "libxtn_probe.so"))
+        probe.xtn_probe_copy.restype = ctypes.c_double
+        probe.xtn_probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int32]
+        ms = probe.xtn_probe_copy(args.probe_copy, 5)
+        print(json.dumps({"probe_copy_bytes": args.probe_copy, "ms": ms,
+                          "GBps": 2 * args.probe_copy / (ms / 1e3) / 1e9 if ms > 0 else None}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs the search on a single MI355X "
               "(the multi-GPU sharding is a design, not a measured path)", file=sys.stderr)
@@ -144,7 +237,11 @@ def main():
                                 "end incl. process start and parsing, GPU unrecorded "
                                 "(data/gpu_benchmark.csv:42); not the same measurement"},
     }
+    if not args.no_batched:
+        result["batched"] = batched_vs_plain(eng, offsets, args.batched_sets, seed)
     eng.close()
+    if not args.no_served:
+        result["served"] = served(args.served_index, args.served_queries, args.served_rate, 5)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(chars, offsets, d, measure, args.cpu_sample)
     print(json.dumps(result))

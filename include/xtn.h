@@ -131,7 +131,52 @@ int xtn_run_nn_resident(xtn_handle* h, int64_t* out_len);
 /* copies up to cap triples of the last xtn_run_nn_resident to host memory */
 int xtn_copy_nn_output(xtn_handle* h, xtn_int2* pairs, int8_t* distances, int64_t cap);
 
+/* Many independent NN searches over the rows set by xtn_set_sequences (no
+ * reference counterpart; the reference runs one input per process). Query set
+ * s is the row range [set_offsets[s], set_offsets[s+1]); pairs are only ever
+ * reported inside a set. batched == 1 runs every set through one pipeline
+ * pass (keys salted with the set, candidates across sets dropped at
+ * verification); batched == 0 runs one pass per set, as separate reference
+ * invocations would. The callback receives each set with output once, in set
+ * order, with set-local row indexes in ascending (x, y) order; callback == NULL
+ * leaves the results on the device (timing). */
+typedef void (*xtn_set_callback)(void* user, int32_t set, const xtn_output* out);
+int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets, int32_t batched,
+                     xtn_set_callback callback, void* user);
+
 int xtn_get_stats(xtn_handle* h, xtn_stats* out);
+
+/* ---- served form (no reference counterpart; SURVEY's query-vs-index mode).
+ * A server keeps the sorted deletion index of n index sequences resident in
+ * HBM and answers query sets against it: every (query row, index row) pair
+ * within the distance. `slots` worker streams run admitted queries
+ * concurrently; a query is admitted, in arrival order, while the estimated
+ * HBM of the queries in flight stays within hbm_budget_bytes. */
+typedef struct {
+	int32_t distance;          /* >= 1 */
+	int32_t measure;           /* XTN_LEVENSHTEIN or XTN_HAMMING */
+	int32_t device;
+	int32_t slots;             /* concurrent queries (0: 4) */
+	int64_t hbm_budget_bytes;  /* admission budget (0: a quarter of free HBM) */
+	int32_t max_query_rows;    /* largest query set accepted (0: 1 << 20) */
+	int32_t reserved;
+} xtn_server_config;
+
+typedef struct xtn_server xtn_server;
+
+void xtn_server_config_init(xtn_server_config* cfg);
+/* builds the index (deletion variants of the index sequences, sorted by key) */
+int xtn_server_open(xtn_server** out, const xtn_server_config* cfg, const char* chars,
+                    const uint32_t* offsets, int32_t n);
+/* copies the query set and queues it; returns at once with a ticket */
+int xtn_server_submit(xtn_server* s, const char* chars, const uint32_t* offsets, int32_t n,
+                      uint64_t* ticket);
+/* waits for a ticket; one callback with index_pairs = (query row, index row)
+ * ascending and pairwise_distances; *latency_ms = submit to completion */
+int xtn_server_wait(xtn_server* s, uint64_t ticket, xtn_callback callback, void* user,
+                    double* latency_ms);
+void xtn_server_close(xtn_server* s);
+const char* xtn_server_last_error(xtn_server* s);
 
 /* profile == 1 only: launches, summed HIP-event duration (ms) and summed
  * algorithmic HBM bytes of one kernel over the runs since the last reset. */

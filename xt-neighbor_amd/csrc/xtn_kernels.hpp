@@ -63,13 +63,13 @@ __device__ __forceinline__ uint64_t variant_count(int L, int d) {
 // (kernel.cu:33-57 counts the same sum of binomials)
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
                                               const uint32_t* __restrict__ offs,
-                                              const uint32_t* __restrict__ seq_map, int32_t n,
-                                              int distance, uint64_t* __restrict__ pk,
+                                              const uint32_t* __restrict__ seq_map, uint32_t row_base,
+                                              int32_t n, int distance, uint64_t* __restrict__ pk,
                                               uint64_t* __restrict__ vcount) {
 	const int32_t i = blockIdx.x * 256 + threadIdx.x;
 	if (i >= n)
 		return;
-	const uint32_t o = seq_map ? seq_map[i] : (uint32_t)i;
+	const uint32_t o = seq_map ? seq_map[i] : row_base + (uint32_t)i;
 	const uint32_t s = offs[o];
 	const uint32_t len = offs[o + 1] - s;
 	const int L = len < (uint32_t)MAX_PREFIX ? (int)len : MAX_PREFIX;
@@ -181,6 +181,7 @@ __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
 constexpr int GEN_NT = 256;
 constexpr int GEN_IT = 16;  // GEN_TILE == RS_TILE: a generator tile is a radix tile
 constexpr int GEN_TILE = GEN_NT * GEN_IT;
+constexpr int GEN_LDS_SEQS = 1023;  // sequences of a tile whose offsets are staged in LDS
 
 enum GenMode { GEN_DIRECT = 0, GEN_HIST = 1, GEN_APPEND = 2 };
 
@@ -198,6 +199,7 @@ struct GenArgs {
 	int64_t cap;
 	unsigned long long* part_hist;  // GEN_HIST: 2^part_bits counters
 	unsigned long long* append;     // GEN_APPEND cursor
+	const uint32_t* set_of;         // nullable: query set of each id, salted into the key
 	uint32_t* digit_hist;           // GEN_DIRECT, nullable: per-tile counts of the
 	int64_t ntiles;                 // 8-bit digit at digit_shift, laid out as
 	int32_t digit_shift;            // k_radix_hist writes them
@@ -222,7 +224,7 @@ __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n,
 // writes a strided run per lane).
 template <int MODE>
 __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
-	__shared__ uint64_t s_voff[GEN_TILE + 2];
+	__shared__ uint64_t s_voff[GEN_LDS_SEQS + 1];
 	__shared__ uint32_t s_binom[19 * 19];
 	__shared__ u128 s_hm[MAX_PREFIX], s_lm[MAX_PREFIX];
 	__shared__ int64_t s_lo;
@@ -256,16 +258,27 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			s_hist[i] = 0;
 	}
 	if (t == 0) {
-		const int64_t lo = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)r0) - 1;
-		const int64_t hi = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)(r1 - 1)) - 1;
+		// voff[n] == V, so both searches land in [0, n-1]; the clamps only
+		// keep a caller's inconsistent V from reading past voff
+		int64_t lo = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)r0) - 1;
+		int64_t hi = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)(r1 - 1)) - 1;
+		lo = lo < a.n - 1 ? lo : a.n - 1;
+		hi = hi < a.n - 1 ? hi : a.n - 1;
+		hi = hi > lo ? hi : lo;
+		hi = hi - lo + 1 <= GEN_TILE ? hi : lo + GEN_TILE - 1;
 		s_lo = lo;
 		s_cnt = (int)(hi - lo + 1);
 	}
 	__syncthreads();
 	const int64_t slo = s_lo;
 	const int cnt = s_cnt;  // sequences touching this tile, <= GEN_TILE
-	for (int i = t; i <= cnt; i += GEN_NT)
-		s_voff[i] = a.voff[slo + i];
+	// tiles of more than GEN_LDS_SEQS (very short) sequences search voff in
+	// global memory instead of the LDS copy
+	const bool lds = cnt <= GEN_LDS_SEQS;
+	const uint64_t* vo = lds ? s_voff : a.voff + slo;
+	if (lds)
+		for (int i = t; i <= cnt; i += GEN_NT)
+			s_voff[i] = a.voff[slo + i];
 	__syncthreads();
 
 	const uint64_t idmask = (1ull << a.idb) - 1ull;
@@ -280,28 +293,31 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		if (valid) {
 			// records of a lane only move forward: advance the sequence cursor
 			if (it == 0) {
-				int lo = 0, hi = cnt;  // largest idx with s_voff[idx] <= r
+				int lo = 0, hi = cnt;  // largest idx with vo[idx] <= r
 				while (lo < hi) {
 					int mid = (lo + hi + 1) >> 1;
-					if (s_voff[mid] <= (uint64_t)r)
+					if (vo[mid] <= (uint64_t)r)
 						lo = mid;
 					else
 						hi = mid - 1;
 				}
 				idx = lo;
 			} else {
-				while (idx + 1 < cnt && s_voff[idx + 1] <= (uint64_t)r)
+				while (idx + 1 < cnt && vo[idx + 1] <= (uint64_t)r)
 					++idx;
 			}
 			const int64_t sid = slo + idx;
-			const uint32_t k = (uint32_t)((uint64_t)r - s_voff[idx]);
+			const uint32_t k = (uint32_t)((uint64_t)r - vo[idx]);
 			const uint64_t lo64 = a.pk[2 * sid];
 			const uint64_t hiw = a.pk[2 * sid + 1];
 			const int L = (int)(hiw >> 56);
 			const int m = a.distance < L ? a.distance : L;
 			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
 			const u128 v = unrank_variant(p, L, m, k, s_binom, s_hm, s_lm);
-			rec = (hash_variant(v) & ~idmask) | ((uint64_t)sid & idmask);
+			uint64_t hv = hash_variant(v);
+			if (a.set_of)  // variants of different query sets get different keys
+				hv ^= mix64(((uint64_t)a.set_of[sid] + 1ull) * 0x9E3779B97F4A7C15ull);
+			rec = (hv & ~idmask) | ((uint64_t)sid & idmask);
 		}
 		if (MODE == GEN_DIRECT) {
 			if (valid) {
@@ -516,16 +532,20 @@ struct BucketArgs {
 
 constexpr int BK_BIN_BITS = 12;
 constexpr int BK_BINS = 1 << BK_BIN_BITS;
-constexpr int BK_BINS_PER_THREAD = BK_BINS / RS_NT;
+constexpr int BK_NT = 512;
+constexpr uint32_t BK_NONE = 0xFFFFFFFFu;
 
-// One workgroup per bucket: a counting sort of the bucket's records into
-// 4096 LDS bins by 12 more hash bits (LDS atomics; the order inside a bin
-// does not matter), then every record pair inside a bin that shares the full
-// stored hash and holds two different ids is emitted.
-__global__ __launch_bounds__(RS_NT) void k_bucket(BucketArgs a) {
+// One workgroup per bucket of <= RS_TILE records: every record is pushed on
+// the LDS list of its bin (12 more hash bits, one atomicExch each), then each
+// record walks the records pushed before it in its bin and emits the pairs
+// that share the full stored hash and hold two different ids. Every unordered
+// pair of a bin is visited exactly once; the emission order is irrelevant
+// because the pairs are sorted afterwards.
+__global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	__shared__ uint64_t s_keys[RS_TILE];
-	__shared__ uint32_t s_bin[BK_BINS];
-	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+	__shared__ uint32_t s_next[RS_TILE];
+	__shared__ uint32_t s_head[BK_BINS];
+	__shared__ uint32_t s_scan[BK_NT / WAVE + 1];
 	__shared__ unsigned long long s_base;
 	const int t = threadIdx.x;
 	const int b = blockIdx.x;
@@ -542,72 +562,35 @@ __global__ __launch_bounds__(RS_NT) void k_bucket(BucketArgs a) {
 		return;
 	}
 	const uint32_t bmask = (1u << a.bin_bits) - 1u;
-	for (int i = t; i < BK_BINS; i += RS_NT)
-		s_bin[i] = 0;
+	for (int i = t; i < BK_BINS; i += BK_NT)
+		s_head[i] = BK_NONE;
+	for (int j = t; j < cnt; j += BK_NT)
+		s_keys[j] = a.rec[s + j];
 	__syncthreads();
-	uint64_t k[RS_IT];
-	uint32_t slot_in_bin[RS_IT];
-#pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
-		const int j = r * RS_NT + t;
-		if (j < cnt) {
-			k[r] = a.rec[s + j];
-			slot_in_bin[r] = atomicAdd(&s_bin[(uint32_t)(k[r] >> a.bin_shift) & bmask], 1u);
+	for (int j = t; j < cnt; j += BK_NT)
+		s_next[j] = atomicExch(&s_head[(uint32_t)(s_keys[j] >> a.bin_shift) & bmask], (uint32_t)j);
+	__syncthreads();
+	uint32_t count = 0;
+	for (int j = t; j < cnt; j += BK_NT) {
+		const uint64_t kj = s_keys[j];
+		for (uint32_t k = s_next[j]; k != BK_NONE; k = s_next[k]) {
+			uint64_t p;
+			count += pair_of(kj, s_keys[k], a.idb, &p) ? 1u : 0u;
 		}
 	}
-	__syncthreads();
-	// exclusive scan over the bins, BK_BINS_PER_THREAD consecutive bins per thread
-	uint32_t c[BK_BINS_PER_THREAD];
-	uint32_t sum = 0;
-#pragma unroll
-	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
-		c[i] = s_bin[t * BK_BINS_PER_THREAD + i];
-		sum += c[i];
-	}
-	uint32_t run = block_excl_scan<RS_NT>(sum, s_scan, (uint32_t*)nullptr);
-	uint32_t count = 0;
-#pragma unroll
-	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
-		s_bin[t * BK_BINS_PER_THREAD + i] = run;
-		run += c[i];
-	}
-	__syncthreads();
-#pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
-		const int j = r * RS_NT + t;
-		if (j < cnt)
-			s_keys[s_bin[(uint32_t)(k[r] >> a.bin_shift) & bmask] + slot_in_bin[r]] = k[r];
-	}
-	__syncthreads();
-	// bins of this thread: [start, start + c[i]); s_bin holds the starts
-#pragma unroll
-	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
-		if (c[i] < 2)
-			continue;
-		const uint32_t b0 = s_bin[t * BK_BINS_PER_THREAD + i], b1 = b0 + c[i];
-		for (uint32_t x = b0; x < b1; ++x)
-			for (uint32_t y = x + 1; y < b1; ++y) {
-				uint64_t p;
-				count += pair_of(s_keys[x], s_keys[y], a.idb, &p) ? 1u : 0u;
-			}
-	}
-	unsigned long long slot = block_append<RS_NT>(&a.ctr->n_cand, count, s_scan, &s_base);
+	unsigned long long slot = block_append<BK_NT>(&a.ctr->n_cand, count, s_scan, &s_base);
 	if (!count)
 		return;
-#pragma unroll
-	for (int i = 0; i < BK_BINS_PER_THREAD; ++i) {
-		if (c[i] < 2)
-			continue;
-		const uint32_t b0 = s_bin[t * BK_BINS_PER_THREAD + i], b1 = b0 + c[i];
-		for (uint32_t x = b0; x < b1; ++x)
-			for (uint32_t y = x + 1; y < b1; ++y) {
-				uint64_t p;
-				if (pair_of(s_keys[x], s_keys[y], a.idb, &p)) {
-					if ((int64_t)slot < a.pcap)
-						a.pairs[slot] = p;
-					++slot;
-				}
+	for (int j = t; j < cnt; j += BK_NT) {
+		const uint64_t kj = s_keys[j];
+		for (uint32_t k = s_next[j]; k != BK_NONE; k = s_next[k]) {
+			uint64_t p;
+			if (pair_of(kj, s_keys[k], a.idb, &p)) {
+				if ((int64_t)slot < a.pcap)
+					a.pairs[slot] = p;
+				++slot;
 			}
+		}
 	}
 }
 
@@ -772,7 +755,8 @@ __device__ __forceinline__ int seq_distance(const uint8_t* a, int la, const uint
 __global__ __launch_bounds__(256) void k_verify(const uint64_t* __restrict__ pairs, int64_t n,
                                                 int idb, const uint8_t* __restrict__ chars,
                                                 const uint32_t* __restrict__ offs,
-                                                const uint32_t* __restrict__ seq_map, int distance,
+                                                const uint32_t* __restrict__ seq_map, uint32_t row_base,
+                                                const uint32_t* __restrict__ set_of, int distance,
                                                 int measure, uint32_t* __restrict__ flags,
                                                 int8_t* __restrict__ dist,
                                                 RunCtrs* __restrict__ ctr) {
@@ -786,13 +770,15 @@ __global__ __launch_bounds__(256) void k_verify(const uint64_t* __restrict__ pai
 		if (uniq) {
 			const uint64_t mask = (1ull << idb) - 1ull;
 			const uint32_t i = (uint32_t)(p >> idb), j = (uint32_t)(p & mask);
-			const uint32_t oi = seq_map ? seq_map[i] : i;
-			const uint32_t oj = seq_map ? seq_map[j] : j;
-			const uint32_t si = offs[oi], sj = offs[oj];
-			const int li = (int)(offs[oi + 1] - si), lj = (int)(offs[oj + 1] - sj);
-			const int d = seq_distance(chars + si, li, chars + sj, lj, distance, measure);
-			keep = d <= distance ? 1u : 0u;
-			dv = (int8_t)d;
+			const uint32_t oi = seq_map ? seq_map[i] : row_base + i;
+			const uint32_t oj = seq_map ? seq_map[j] : row_base + j;
+			if (!set_of || set_of[i] == set_of[j]) {
+				const uint32_t si = offs[oi], sj = offs[oj];
+				const int li = (int)(offs[oi + 1] - si), lj = (int)(offs[oj + 1] - sj);
+				const int d = seq_distance(chars + si, li, chars + sj, lj, distance, measure);
+				keep = d <= distance ? 1u : 0u;
+				dv = (int8_t)d;
+			}
 		}
 	}
 	__shared__ unsigned long long s_tmp[256 / WAVE + 1];

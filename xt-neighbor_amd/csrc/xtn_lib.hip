@@ -13,6 +13,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstddef>
 #include <cstdarg>
 #include <cstdio>
@@ -24,6 +29,7 @@
 #include "../../include/xtn.h"
 #include "xtn_kernels.hpp"
 #include "xtn_overlap.hpp"
+#include "xtn_serve.hpp"
 
 using namespace xtn;
 
@@ -77,7 +83,7 @@ struct xtn_handle {
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAcc> prof;
 	std::vector<PendingEvent> spans;  // stage timing, always on
-	DBuf bstart, ovf;
+	DBuf bstart, ovf, set_of;
 	int64_t last_pairs = 0;
 };
 
@@ -369,7 +375,7 @@ double now_ms() {
 
 // ----------------------------------------------------------- NN pipeline --
 
-constexpr int64_t BUCKET_TARGET = 2800;  // mean records per LDS bucket
+constexpr int64_t BUCKET_TARGET = 3500;  // mean records per LDS bucket (<= 4096 fit)
 constexpr int KEY_BITS = 32;             // hash bits that order records into runs
 
 inline unsigned long long* ctr_field(xtn_handle* h, size_t offset) {
@@ -449,7 +455,8 @@ int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, i
 // Runs the candidate search and verification over `nseq` ids. seq_map (device,
 // nullable) maps an id to its input row. Leaves sorted (x, y, distance)
 // triples in out_pairs/out_dist and their count in h->out_len.
-int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
+int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t row_base = 0,
+                const uint32_t* set_of = nullptr) {
 	hipStream_t s = h->stream;
 	const int d = h->cfg.distance;
 	const int measure = h->cfg.measure;
@@ -474,7 +481,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 		StageScope sg(h, "generate");
 		ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
 		k_pack<<<grid_for(nseq, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
-		                                            seq_map, nseq, d, h->pk.as<uint64_t>(),
+		                                            seq_map, row_base, nseq, d, h->pk.as<uint64_t>(),
 		                                            h->voff.as<uint64_t>());
 		HIP_LAUNCH_CHECK(h);
 	}
@@ -520,6 +527,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 	ga.idb = idb;
 	ga.part_bits = pbits;
 	ga.V = (int64_t)V;
+	ga.set_of = set_of;
 	const unsigned gen_grid = grid_for((int64_t)V, GEN_TILE);
 
 	std::vector<unsigned long long> part_count(nparts, 0);
@@ -565,9 +573,14 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 			continue;
 		ga.out = h->recs.as<uint64_t>();
 		ga.cap = cap_recs;
-		// bucket bits: the B hash bits right below the chunk bits
-		int B = nrec <= BUCKET_TARGET ? 0 : (nrec <= 256 * BUCKET_TARGET ? 8 : 16);
+		// bucket bits: the B hash bits right below the chunk bits, B such that
+		// buckets hold about BUCKET_TARGET records; the sort orders the
+		// records by the next multiple of 8 bits
+		int B = 0;
+		while (B < 16 && (nrec >> B) > BUCKET_TARGET)
+			++B;
 		B = std::min(B, K);
+		const int SB = (B + 7) / 8 * 8;
 		// the single-chunk generator also counts the first sort digit per tile
 		const bool fused_hist = nparts == 1 && B > 0;
 		const int64_t rtiles = (nrec + RS_TILE - 1) / RS_TILE;
@@ -575,7 +588,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 			TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
 			ga.digit_hist = h->hist.as<uint32_t>();
 			ga.ntiles = rtiles;
-			ga.digit_shift = hi - B;
+			ga.digit_shift = hi - SB;
 		} else {
 			ga.digit_hist = nullptr;
 		}
@@ -596,7 +609,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 		}
 		if (B > 0) {
 			StageScope sg(h, "sort");
-			TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, hi - B, hi, "sort_rec",
+			TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, hi - SB, hi, "sort_rec",
 			               fused_hist));
 		}
 		const int nb = 1 << B;
@@ -625,7 +638,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 			ba.pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
 			{
 				ProfScope ps(h, "bucket", (double)nrec * 8.0);
-				k_bucket<<<(unsigned)nb, RS_NT, 0, s>>>(ba);
+				k_bucket<<<(unsigned)nb, BK_NT, 0, s>>>(ba);
 				HIP_LAUNCH_CHECK(h);
 			}
 			TRY(read_ctrs(h, &rc));
@@ -667,7 +680,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map) {
 			ProfScope ps(h, "verify", (double)n_pairs * 13.0);
 			k_verify<<<grid_for(n_pairs, 256), 256, 0, s>>>(
 			    h->pairs.as<uint64_t>(), n_pairs, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
-			    seq_map, d, measure, h->flags.as<uint32_t>(), h->dist.as<int8_t>(), h->ctrs.as<RunCtrs>());
+			    seq_map, row_base, set_of, d, measure, h->flags.as<uint32_t>(), h->dist.as<int8_t>(),
+			    h->ctrs.as<RunCtrs>());
 			HIP_LAUNCH_CHECK(h);
 		}
 		TRY(scan_excl<uint32_t>(h, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), n_pairs,
@@ -1078,6 +1092,100 @@ int xtn_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, 
 	return rc;
 }
 
+static int copy_output(xtn_handle* h, std::vector<xtn_int2>& pairs, std::vector<int8_t>& dist) {
+	const int64_t n = h->out_len;
+	pairs.resize((size_t)std::max<int64_t>(n, 1));
+	dist.resize((size_t)std::max<int64_t>(n, 1));
+	if (n > 0) {
+		HIP_TRY(h, hipMemcpyAsync(pairs.data(), h->out_pairs.p, sizeof(xtn_int2) * (size_t)n,
+		                          hipMemcpyDeviceToHost, h->stream));
+		HIP_TRY(h, hipMemcpyAsync(dist.data(), h->out_dist.p, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+		HIP_TRY(h, hipStreamSynchronize(h->stream));
+	}
+	return XTN_OK;
+}
+
+int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets, int32_t batched,
+                     xtn_set_callback callback, void* user) {
+	if (!h || n_sets < 0 || (n_sets > 0 && !set_offsets))
+		return h ? fail(h, XTN_ERR_ARG, "invalid query sets") : XTN_ERR_ARG;
+	if (!h->offs.p)
+		return fail(h, XTN_ERR_STATE, "no sequences set");
+	for (int32_t k = 0; k < n_sets; ++k)
+		if (set_offsets[k + 1] < set_offsets[k])
+			return fail(h, XTN_ERR_ARG, "set offsets must be non-decreasing");
+	if (n_sets > 0 && set_offsets[n_sets] > (uint32_t)h->n)
+		return fail(h, XTN_ERR_ARG, "query sets exceed the %d rows", h->n);
+	HIP_TRY(h, hipSetDevice(h->cfg.device));
+	memset(&h->stats, 0, sizeof(h->stats));
+	if (n_sets == 0)
+		return XTN_OK;
+	const double t0 = now_ms();
+	xtn_stats tot;
+	memset(&tot, 0, sizeof(tot));
+	auto accumulate = [&]() {
+		const xtn_stats& st = h->stats;
+		tot.n_sequences += st.n_sequences;
+		tot.n_variants += st.n_variants;
+		tot.n_chunks += st.n_chunks;
+		tot.n_runs += st.n_runs;
+		tot.n_candidates += st.n_candidates;
+		tot.n_unique += st.n_unique;
+		tot.n_output += st.n_output;
+		tot.ms_generate += st.ms_generate;
+		tot.ms_sort += st.ms_sort;
+		tot.ms_pairs += st.ms_pairs;
+		tot.ms_verify += st.ms_verify;
+	};
+	std::vector<xtn_int2> hp;
+	std::vector<int8_t> hd;
+	if (batched) {
+		const uint32_t r0 = set_offsets[0], r1 = set_offsets[n_sets];
+		std::vector<uint32_t> set_of((size_t)std::max<uint32_t>(r1 - r0, 1));
+		for (int32_t k = 0; k < n_sets; ++k)
+			for (uint32_t r = set_offsets[k]; r < set_offsets[k + 1]; ++r)
+				set_of[r - r0] = (uint32_t)k;
+		TRY(ensure(h, h->set_of, sizeof(uint32_t) * set_of.size()));
+		HIP_TRY(h, hipMemcpyAsync(h->set_of.p, set_of.data(), sizeof(uint32_t) * set_of.size(),
+		                          hipMemcpyHostToDevice, h->stream));
+		TRY(nn_pipeline(h, (int32_t)(r1 - r0), nullptr, r0, h->set_of.as<uint32_t>()));
+		accumulate();
+		if (callback && h->out_len > 0) {
+			TRY(copy_output(h, hp, hd));
+			// the sorted output is grouped by set, sets in order
+			int64_t i = 0;
+			const int64_t n = h->out_len;
+			while (i < n) {
+				const uint32_t k = set_of[(uint32_t)hp[i].x];
+				const int32_t base = (int32_t)(set_offsets[k] - r0);
+				int64_t j = i;
+				while (j < n && set_of[(uint32_t)hp[j].x] == k) {
+					hp[j].x -= base;
+					hp[j].y -= base;
+					++j;
+				}
+				xtn_output out{hp.data() + i, hd.data() + i, nullptr, j - i};
+				callback(user, (int32_t)k, &out);
+				i = j;
+			}
+		}
+	} else {
+		for (int32_t k = 0; k < n_sets; ++k) {
+			const uint32_t a = set_offsets[k], b = set_offsets[k + 1];
+			TRY(nn_pipeline(h, (int32_t)(b - a), nullptr, a, nullptr));
+			accumulate();
+			if (callback && h->out_len > 0) {
+				TRY(copy_output(h, hp, hd));
+				xtn_output out{hp.data(), hd.data(), nullptr, h->out_len};
+				callback(user, k, &out);
+			}
+		}
+	}
+	tot.ms_total = now_ms() - t0;
+	h->stats = tot;
+	return XTN_OK;
+}
+
 int xtn_get_stats(xtn_handle* h, xtn_stats* out) {
 	if (!h || !out)
 		return XTN_ERR_ARG;
@@ -1125,5 +1233,391 @@ const char* xtn_status_string(int status) {
 }
 
 const char* xtn_version(void) { return "2.0.0-mi355x"; }
+
+}  // extern "C"
+
+// ============================================================ server ==
+
+namespace {
+
+struct ServeRequest {
+	uint64_t ticket = 0;
+	std::vector<char> chars;
+	std::vector<uint32_t> offs;
+	int32_t n = 0;
+	int64_t variants = 0;
+	int64_t est_bytes = 0;
+	double t_submit = 0;
+};
+
+struct ServeResult {
+	std::vector<xtn_int2> pairs;
+	std::vector<int8_t> dist;
+	double latency_ms = 0;
+	int status = XTN_OK;
+	std::string err;
+};
+
+uint64_t host_variant_count(uint32_t len, int d) {
+	const int L = len < 18 ? (int)len : 18;
+	const int m = d < L ? d : L;
+	uint64_t c = 1, total = 1;
+	for (int j = 1; j <= m; ++j) {
+		c = c * (uint64_t)(L - j + 1) / (uint64_t)j;
+		total += c;
+	}
+	return total;
+}
+
+}  // namespace
+
+struct xtn_server {
+	xtn_server_config cfg;
+	xtn_handle* base = nullptr;  // owns the index: chars/offs, recs (sorted), bstart (directory)
+	int32_t n_index = 0;
+	int idb = 0;
+	int64_t v_index = 0;
+	const uint64_t* index = nullptr;
+	const uint32_t* dir = nullptr;
+	std::vector<xtn_handle*> workers;
+	std::vector<std::thread> threads;
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::deque<ServeRequest> queue;
+	std::unordered_map<uint64_t, ServeResult> done;
+	uint64_t next_ticket = 1;
+	int64_t inflight = 0;
+	int64_t budget = 0;
+	bool stop = false;
+	std::string err;
+};
+
+namespace {
+
+// one query on a worker handle: variants, index lookups, candidate sort and
+// unique, verification, compaction, download
+int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResult* res) {
+	hipStream_t s = w->stream;
+	const int32_t n = rq.n;
+	if (n == 0)
+		return XTN_OK;
+	const int idb = sv->idb;
+	const int d = sv->cfg.distance;
+	const uint32_t total = rq.offs[n];
+	TRY(ensure(w, w->chars, (size_t)total + 16));
+	TRY(ensure(w, w->offs, sizeof(uint32_t) * ((size_t)n + 1)));
+	if (total)
+		HIP_TRY(w, hipMemcpyAsync(w->chars.p, rq.chars.data(), total, hipMemcpyHostToDevice, s));
+	HIP_TRY(w, hipMemcpyAsync(w->offs.p, rq.offs.data(), sizeof(uint32_t) * ((size_t)n + 1),
+	                          hipMemcpyHostToDevice, s));
+	TRY(ensure(w, w->pk, sizeof(uint64_t) * 2 * (size_t)n));
+	TRY(ensure(w, w->voff, sizeof(uint64_t) * ((size_t)n + 1)));
+	TRY(ensure(w, w->ctrs, sizeof(RunCtrs)));
+	k_pack<<<grid_for(n, 256), 256, 0, s>>>(w->chars.as<uint8_t>(), w->offs.as<uint32_t>(), nullptr, 0, n, d,
+	                                         w->pk.as<uint64_t>(), w->voff.as<uint64_t>());
+	HIP_LAUNCH_CHECK(w);
+	TRY(scan_excl<uint64_t>(w, w->voff.as<uint64_t>(), w->voff.as<uint64_t>(), n, w->voff.as<uint64_t>() + n));
+	const int64_t V = rq.variants;  // the same closed form the device scan computes
+	TRY(ensure(w, w->recs, sizeof(uint64_t) * (size_t)V));
+	GenArgs ga{};
+	ga.pk = w->pk.as<uint64_t>();
+	ga.voff = w->voff.as<uint64_t>();
+	ga.n = n;
+	ga.distance = d;
+	ga.idb = idb;
+	ga.V = V;
+	ga.out = w->recs.as<uint64_t>();
+	ga.cap = V;
+	k_gen<GEN_DIRECT><<<grid_for(V, GEN_TILE), GEN_NT, 0, s>>>(ga);
+	HIP_LAUNCH_CHECK(w);
+	int64_t cap = std::max<int64_t>({w->last_pairs + w->last_pairs / 4, V / 2, 1 << 16});
+	unsigned long long found = 0;
+	for (int attempt = 0; attempt < 2; ++attempt) {
+		TRY(ensure(w, w->pairs, sizeof(uint64_t) * (size_t)cap));
+		k_fill_u64<<<1, 256, 0, s>>>(&w->ctrs.as<RunCtrs>()->n_cand, 1, 0ull);
+		k_match<<<grid_for(V, 256), 256, 0, s>>>(w->recs.as<uint64_t>(), V, sv->index, sv->dir, idb,
+		                                          w->pairs.as<uint64_t>(), cap, &w->ctrs.as<RunCtrs>()->n_cand);
+		HIP_LAUNCH_CHECK(w);
+		HIP_TRY(w, hipMemcpyAsync(&found, &w->ctrs.as<RunCtrs>()->n_cand, sizeof(found), hipMemcpyDeviceToHost, s));
+		HIP_TRY(w, hipStreamSynchronize(s));
+		if ((int64_t)found <= cap)
+			break;
+		if (found > (unsigned long long)INT32_MAX)
+			return fail(w, XTN_ERR_CAPACITY, "query candidates exceed 2^31");
+		cap = (int64_t)found + (int64_t)found / 4;
+	}
+	const int64_t np = (int64_t)found;
+	w->last_pairs = np;
+	if (np == 0)
+		return XTN_OK;
+	TRY(ensure(w, w->pairs_alt, sizeof(uint64_t) * (size_t)np));
+	TRY(radix_sort(w, w->pairs, w->pairs_alt, nullptr, nullptr, np, 0, 2 * idb, "sort_pairs"));
+	TRY(ensure(w, w->flags, sizeof(uint32_t) * (size_t)np));
+	TRY(ensure(w, w->pos, sizeof(uint32_t) * ((size_t)np + 1)));
+	TRY(ensure(w, w->dist, (size_t)np));
+	k_verify_cross<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->chars.as<uint8_t>(),
+	                                                  w->offs.as<uint32_t>(), sv->base->chars.as<uint8_t>(),
+	                                                  sv->base->offs.as<uint32_t>(), d, sv->cfg.measure,
+	                                                  w->flags.as<uint32_t>(), w->dist.as<int8_t>());
+	HIP_LAUNCH_CHECK(w);
+	TRY(scan_excl<uint32_t>(w, w->flags.as<uint32_t>(), w->pos.as<uint32_t>(), np, w->pos.as<uint32_t>() + np));
+	uint32_t nout = 0;
+	HIP_TRY(w, hipMemcpyAsync(&nout, w->pos.as<uint32_t>() + np, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_TRY(w, hipStreamSynchronize(s));
+	TRY(ensure(w, w->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
+	TRY(ensure(w, w->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
+	k_emit_nn<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->flags.as<uint32_t>(),
+	                                             w->pos.as<uint32_t>(), w->dist.as<int8_t>(),
+	                                             w->out_pairs.as<int2>(), w->out_dist.as<int8_t>());
+	HIP_LAUNCH_CHECK(w);
+	res->pairs.resize(nout);
+	res->dist.resize(nout);
+	if (nout) {
+		HIP_TRY(w, hipMemcpyAsync(res->pairs.data(), w->out_pairs.p, sizeof(xtn_int2) * nout,
+		                          hipMemcpyDeviceToHost, s));
+		HIP_TRY(w, hipMemcpyAsync(res->dist.data(), w->out_dist.p, nout, hipMemcpyDeviceToHost, s));
+	}
+	HIP_TRY(w, hipStreamSynchronize(s));
+	return XTN_OK;
+}
+
+void serve_worker(xtn_server* sv, int slot) {
+	xtn_handle* w = sv->workers[slot];
+	(void)hipSetDevice(sv->cfg.device);
+	for (;;) {
+		ServeRequest rq;
+		{
+			std::unique_lock<std::mutex> lk(sv->mu);
+			// FIFO admission: the head of the queue starts once its estimate
+			// fits the budget, or when nothing else is in flight
+			sv->cv_work.wait(lk, [&] {
+				return sv->stop || (!sv->queue.empty() &&
+				                    (sv->inflight == 0 || sv->inflight + sv->queue.front().est_bytes <= sv->budget));
+			});
+			if (sv->queue.empty())
+				return;  // stop requested and nothing left
+			rq = std::move(sv->queue.front());
+			sv->queue.pop_front();
+			sv->inflight += rq.est_bytes;
+		}
+		ServeResult res;
+		res.status = serve_query(sv, w, rq, &res);
+		if (res.status != XTN_OK)
+			res.err = w->err;
+		res.latency_ms = now_ms() - rq.t_submit;
+		{
+			std::lock_guard<std::mutex> lk(sv->mu);
+			sv->inflight -= rq.est_bytes;
+			sv->done[rq.ticket] = std::move(res);
+		}
+		sv->cv_done.notify_all();
+		sv->cv_work.notify_all();
+	}
+}
+
+}  // namespace
+
+extern "C" {
+
+void xtn_server_config_init(xtn_server_config* cfg) {
+	if (!cfg)
+		return;
+	memset(cfg, 0, sizeof(*cfg));
+	cfg->distance = 1;
+	cfg->measure = XTN_LEVENSHTEIN;
+	cfg->slots = 4;
+}
+
+int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const char* chars,
+                    const uint32_t* offsets, int32_t n) {
+	if (!out || !offsets || n < 1)
+		return XTN_ERR_ARG;
+	*out = nullptr;
+	xtn_server_config cfg;
+	xtn_server_config_init(&cfg);
+	if (cfg_in)
+		cfg = *cfg_in;
+	if (cfg.slots <= 0)
+		cfg.slots = 4;
+	if (cfg.slots > 16)
+		cfg.slots = 16;
+	if (cfg.max_query_rows <= 0)
+		cfg.max_query_rows = 1 << 20;
+	xtn_config hc;
+	xtn_config_init(&hc);
+	hc.distance = cfg.distance;
+	hc.measure = cfg.measure;
+	hc.device = cfg.device;
+	xtn_server* sv = new xtn_server();
+	sv->cfg = cfg;
+	int rc = xtn_open(&sv->base, &hc);
+	if (rc != XTN_OK) {
+		delete sv;
+		return rc;
+	}
+	xtn_handle* h = sv->base;
+	rc = xtn_set_sequences(h, chars, offsets, n);
+	if (rc != XTN_OK) {
+		sv->err = h->err;
+		xtn_server_close(sv);
+		return rc;
+	}
+	sv->n_index = n;
+	sv->idb = std::max(bits_for(std::max(n - 1, 1)), bits_for(std::max(cfg.max_query_rows - 1, 1)));
+	if (2 * sv->idb > 62 || 64 - sv->idb < DIR_BITS + 8) {
+		xtn_server_close(sv);
+		return XTN_ERR_CAPACITY;
+	}
+	auto build = [&]() -> int {
+		hipStream_t s = h->stream;
+		TRY(ensure(h, h->pk, sizeof(uint64_t) * 2 * (size_t)n));
+		TRY(ensure(h, h->voff, sizeof(uint64_t) * ((size_t)n + 1)));
+		k_pack<<<grid_for(n, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), nullptr, 0, n,
+		                                         cfg.distance, h->pk.as<uint64_t>(), h->voff.as<uint64_t>());
+		HIP_LAUNCH_CHECK(h);
+		TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), n, h->voff.as<uint64_t>() + n));
+		uint64_t V = 0;
+		HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + n, sizeof(V), hipMemcpyDeviceToHost, s));
+		HIP_TRY(h, hipStreamSynchronize(s));
+		if (V > (uint64_t)INT32_MAX)
+			return fail(h, XTN_ERR_CAPACITY, "index of %llu records exceeds 2^31", (unsigned long long)V);
+		TRY(ensure(h, h->recs, sizeof(uint64_t) * V));
+		TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * V));
+		GenArgs ga{};
+		ga.pk = h->pk.as<uint64_t>();
+		ga.voff = h->voff.as<uint64_t>();
+		ga.n = n;
+		ga.distance = cfg.distance;
+		ga.idb = sv->idb;
+		ga.V = (int64_t)V;
+		ga.out = h->recs.as<uint64_t>();
+		ga.cap = (int64_t)V;
+		k_gen<GEN_DIRECT><<<grid_for((int64_t)V, GEN_TILE), GEN_NT, 0, s>>>(ga);
+		HIP_LAUNCH_CHECK(h);
+		TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, (int64_t)V, sv->idb, 64, "index_sort"));
+		release(h->recs_alt);
+		TRY(ensure(h, h->bstart, sizeof(uint32_t) * (((size_t)1 << DIR_BITS) + 1)));
+		k_dir_bounds<<<grid_for(((int64_t)1 << DIR_BITS) + 1, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), (int64_t)V,
+		                                                                         h->bstart.as<uint32_t>());
+		HIP_LAUNCH_CHECK(h);
+		HIP_TRY(h, hipStreamSynchronize(s));
+		sv->v_index = (int64_t)V;
+		sv->index = h->recs.as<uint64_t>();
+		sv->dir = h->bstart.as<uint32_t>();
+		size_t fr = 0, tot = 0;
+		HIP_TRY(h, hipMemGetInfo(&fr, &tot));
+		sv->budget = cfg.hbm_budget_bytes > 0 ? cfg.hbm_budget_bytes : (int64_t)(fr / 4);
+		return XTN_OK;
+	};
+	rc = build();
+	if (rc != XTN_OK) {
+		sv->err = h->err;
+		xtn_server_close(sv);
+		return rc;
+	}
+	for (int k = 0; k < cfg.slots; ++k) {
+		xtn_handle* w = nullptr;
+		rc = xtn_open(&w, &hc);
+		if (rc != XTN_OK) {
+			xtn_server_close(sv);
+			return rc;
+		}
+		sv->workers.push_back(w);
+	}
+	for (int k = 0; k < cfg.slots; ++k)
+		sv->threads.emplace_back(serve_worker, sv, k);
+	*out = sv;
+	return XTN_OK;
+}
+
+int xtn_server_submit(xtn_server* sv, const char* chars, const uint32_t* offsets, int32_t n, uint64_t* ticket) {
+	if (!sv || !offsets || n < 0 || !ticket)
+		return XTN_ERR_ARG;
+	if (n > sv->cfg.max_query_rows) {
+		sv->err = "query set larger than max_query_rows";
+		return XTN_ERR_CAPACITY;
+	}
+	if (offsets[0] != 0) {
+		sv->err = "offsets[0] must be 0";
+		return XTN_ERR_ARG;
+	}
+	ServeRequest rq;
+	rq.n = n;
+	for (int32_t i = 0; i < n; ++i)
+		if (offsets[i + 1] < offsets[i]) {
+			sv->err = "offsets must be non-decreasing";
+			return XTN_ERR_ARG;
+		}
+	const uint32_t total = offsets[n];
+	for (uint32_t k = 0; k < total; ++k)
+		if (!valid_residue((unsigned char)chars[k])) {
+			sv->err = "invalid char";
+			return XTN_ERROR;
+		}
+	rq.chars.assign(chars, chars + total);
+	rq.offs.assign(offsets, offsets + n + 1);
+	for (int32_t i = 0; i < n; ++i)
+		rq.variants += (int64_t)host_variant_count(offsets[i + 1] - offsets[i], sv->cfg.distance);
+	if (rq.variants > (int64_t)INT32_MAX) {
+		sv->err = "query variants exceed 2^31";
+		return XTN_ERR_CAPACITY;
+	}
+	// records, candidates (about V/2, sort buffer), flags/positions
+	rq.est_bytes = rq.variants * 8 + rq.variants * 12 + (int64_t)n * 64 + (int64_t)total;
+	rq.t_submit = now_ms();
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		rq.ticket = sv->next_ticket++;
+		*ticket = rq.ticket;
+		sv->queue.push_back(std::move(rq));
+	}
+	sv->cv_work.notify_all();
+	return XTN_OK;
+}
+
+int xtn_server_wait(xtn_server* sv, uint64_t ticket, xtn_callback callback, void* user, double* latency_ms) {
+	if (!sv)
+		return XTN_ERR_ARG;
+	ServeResult res;
+	{
+		std::unique_lock<std::mutex> lk(sv->mu);
+		if (ticket == 0 || ticket >= sv->next_ticket) {
+			sv->err = "unknown ticket";
+			return XTN_ERR_ARG;
+		}
+		sv->cv_done.wait(lk, [&] { return sv->done.count(ticket) > 0; });
+		res = std::move(sv->done[ticket]);
+		sv->done.erase(ticket);
+	}
+	if (latency_ms)
+		*latency_ms = res.latency_ms;
+	if (res.status != XTN_OK) {
+		std::lock_guard<std::mutex> lk(sv->mu);
+		sv->err = res.err;
+		return res.status;
+	}
+	xtn_output out{res.pairs.data(), res.dist.data(), nullptr, (int64_t)res.pairs.size()};
+	if (callback)
+		callback(user, &out);
+	return XTN_OK;
+}
+
+void xtn_server_close(xtn_server* sv) {
+	if (!sv)
+		return;
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		sv->stop = true;
+	}
+	sv->cv_work.notify_all();
+	for (auto& t : sv->threads)
+		t.join();
+	for (xtn_handle* w : sv->workers)
+		xtn_close(w);
+	xtn_close(sv->base);
+	delete sv;
+}
+
+const char* xtn_server_last_error(xtn_server* sv) { return sv ? sv->err.c_str() : "null server"; }
 
 }  // extern "C"

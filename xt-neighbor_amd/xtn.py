@@ -35,7 +35,8 @@ EXPORTED = [
     "xtn_config_init", "xtn_open", "xtn_close", "xtn_set_config", "xtn_set_sequences",
     "xtn_perform", "xtn_run_nn_resident", "xtn_copy_nn_output", "xtn_get_stats",
     "xtn_kernel_profile", "xtn_kernel_profile_reset", "xtn_last_error", "xtn_status_string",
-    "xtn_version",
+    "xtn_version", "xtn_perform_sets", "xtn_server_config_init", "xtn_server_open",
+    "xtn_server_submit", "xtn_server_wait", "xtn_server_close", "xtn_server_last_error",
 ]
 
 
@@ -69,6 +70,13 @@ class Config(ctypes.Structure):
                 ("profile", ctypes.c_int32), ("bucket_capacity", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("distance", ctypes.c_int32), ("measure", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("slots", ctypes.c_int32),
+                ("hbm_budget_bytes", ctypes.c_int64), ("max_query_rows", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("n_sequences", ctypes.c_int64), ("n_variants", ctypes.c_int64),
                 ("n_chunks", ctypes.c_int64), ("n_runs", ctypes.c_int64),
@@ -80,6 +88,7 @@ class Stats(ctypes.Structure):
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Output))
+SET_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Output))
 
 _lib = None
 _synth = None
@@ -105,6 +114,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.xtn_run_nn_resident.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     lib.xtn_copy_nn_output.argtypes = [vp, ctypes.POINTER(Int2), ctypes.POINTER(ctypes.c_int8), ctypes.c_int64]
     lib.xtn_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.xtn_perform_sets.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32, ctypes.c_int32,
+                                     SET_CALLBACK, vp]
     lib.xtn_kernel_profile.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.xtn_kernel_profile_reset.argtypes = [vp]
@@ -113,8 +124,22 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.xtn_status_string.argtypes = [ctypes.c_int]
     lib.xtn_status_string.restype = ctypes.c_char_p
     lib.xtn_version.restype = ctypes.c_char_p
+    lib.xtn_server_config_init.argtypes = [ctypes.POINTER(ServerConfig)]
+    lib.xtn_server_config_init.restype = None
+    lib.xtn_server_open.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ServerConfig), ctypes.c_char_p,
+                                    ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32]
+    lib.xtn_server_open.restype = ctypes.c_int
+    lib.xtn_server_submit.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32,
+                                      ctypes.POINTER(ctypes.c_uint64)]
+    lib.xtn_server_submit.restype = ctypes.c_int
+    lib.xtn_server_wait.argtypes = [vp, ctypes.c_uint64, CALLBACK, vp, ctypes.POINTER(ctypes.c_double)]
+    lib.xtn_server_wait.restype = ctypes.c_int
+    lib.xtn_server_close.argtypes = [vp]
+    lib.xtn_server_close.restype = None
+    lib.xtn_server_last_error.argtypes = [vp]
+    lib.xtn_server_last_error.restype = ctypes.c_char_p
     for name in ("xtn_open", "xtn_set_config", "xtn_set_sequences", "xtn_perform",
-                 "xtn_run_nn_resident", "xtn_copy_nn_output", "xtn_get_stats",
+                 "xtn_run_nn_resident", "xtn_copy_nn_output", "xtn_get_stats", "xtn_perform_sets",
                  "xtn_kernel_profile", "xtn_kernel_profile_reset"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
@@ -358,6 +383,23 @@ class Engine:
         self.overlap_calls = calls[0]
         return np.concatenate(parts) if parts else np.zeros((0, 3), np.uint64)
 
+    def nn_sets(self, set_offsets, batched: bool = True, collect: bool = True) -> dict:
+        """{set: int64[n,3] (x, y, dist) set-local} for every set with output"""
+        so = np.ascontiguousarray(set_offsets, dtype=np.uint32)
+        res = {}
+
+        def cb(_user, k, out_p):
+            o = out_p.contents
+            n = o.len
+            p = np.ctypeslib.as_array(ctypes.cast(o.index_pairs, ctypes.POINTER(ctypes.c_int32)),
+                                      shape=(n * 2,)).reshape(n, 2).astype(np.int64)
+            d = np.ctypeslib.as_array(o.pairwise_distances, shape=(n,)).astype(np.int64)
+            res[int(k)] = np.concatenate([p, d[:, None]], axis=1)
+
+        c = SET_CALLBACK(cb) if collect else ctypes.cast(None, SET_CALLBACK)
+        self._check(self.lib.xtn_perform_sets(self.h, _u32p(so), len(so) - 1, 1 if batched else 0, c, None))
+        return res
+
     def stats(self) -> dict:
         s = Stats()
         self._check(self.lib.xtn_get_stats(self.h, ctypes.byref(s)))
@@ -373,3 +415,68 @@ class Engine:
 
     def kernel_profile_reset(self):
         self._check(self.lib.xtn_kernel_profile_reset(self.h))
+
+
+class Server:
+    """A resident deletion index over `index` sequences answering query sets
+    (xtn_server_*). submit() returns a ticket at once; wait(ticket) returns
+    (pairs int32[n,2] = (query row, index row), distances int8[n], latency ms)."""
+
+    def __init__(self, chars: np.ndarray, offsets: np.ndarray, distance: int = 1,
+                 measure: int | str = LEVENSHTEIN, slots: int = 4, hbm_budget_bytes: int = 0,
+                 max_query_rows: int = 0, device: int = 0):
+        self.lib = load_library()
+        cfg = ServerConfig()
+        self.lib.xtn_server_config_init(ctypes.byref(cfg))
+        cfg.distance = distance
+        cfg.measure = {"leven": LEVENSHTEIN, "hamming": HAMMING}.get(measure, measure)
+        cfg.slots = slots
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.max_query_rows = max_query_rows
+        cfg.device = device
+        chars = np.ascontiguousarray(chars, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+        self.s = ctypes.c_void_p()
+        st = self.lib.xtn_server_open(ctypes.byref(self.s), ctypes.byref(cfg),
+                                      chars.ctypes.data_as(ctypes.c_char_p), _u32p(offsets), len(offsets) - 1)
+        if st != 0:
+            raise XtnError(st, "xtn_server_open failed")
+
+    def submit(self, chars: np.ndarray, offsets: np.ndarray) -> int:
+        chars = np.ascontiguousarray(chars, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+        t = ctypes.c_uint64()
+        st = self.lib.xtn_server_submit(self.s, chars.ctypes.data_as(ctypes.c_char_p), _u32p(offsets),
+                                        len(offsets) - 1, ctypes.byref(t))
+        if st != 0:
+            raise XtnError(st, self.lib.xtn_server_last_error(self.s).decode())
+        return t.value
+
+    def wait(self, ticket: int):
+        got = {}
+
+        def cb(_user, out_p):
+            o = out_p.contents
+            n = o.len
+            if n:
+                got["p"] = np.ctypeslib.as_array(ctypes.cast(o.index_pairs, ctypes.POINTER(ctypes.c_int32)),
+                                                 shape=(n * 2,)).reshape(n, 2).copy()
+                got["d"] = np.ctypeslib.as_array(o.pairwise_distances, shape=(n,)).copy()
+
+        c = CALLBACK(cb)
+        lat = ctypes.c_double()
+        st = self.lib.xtn_server_wait(self.s, ticket, c, None, ctypes.byref(lat))
+        if st != 0:
+            raise XtnError(st, self.lib.xtn_server_last_error(self.s).decode())
+        return got.get("p", np.zeros((0, 2), np.int32)), got.get("d", np.zeros(0, np.int8)), lat.value
+
+    def close(self):
+        if self.s:
+            self.lib.xtn_server_close(self.s)
+            self.s = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
