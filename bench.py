@@ -144,6 +144,41 @@ def served(n_index, n_queries, rate_hz, seed):
             "wall_s": wall}
 
 
+def end_to_end(chars, offsets, d, measure):
+    """the reference's published measurement: the CLI on a CSV of the same 1M
+    sequences, process wall time without -o (02D_symdel_large_scale.ipynb
+    cell 23-25) and with -o"""
+    import subprocess
+    import tempfile
+
+    n = len(offsets) - 1
+    tmp = tempfile.mkdtemp(prefix="xtn_bench_")
+    path = os.path.join(tmp, "input.csv")
+    body = chars.tobytes()
+    with open(path, "wb") as f:
+        f.write(b"cdr3\n")
+        f.write(b"\n".join(body[offsets[i]:offsets[i + 1]] for i in range(n)))
+        f.write(b"\n")
+    cmd = [xtn.CLI_PATH, "-i", path, "-n", str(n), "-d", str(d), "-m", "hamming" if measure else "leven"]
+    out = {}
+    for label, extra in (("no_output", []), ("with_output", ["-o", os.path.join(tmp, "out.txt")])):
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = subprocess.run(cmd + extra, capture_output=True)
+            times.append(time.perf_counter() - t0)
+            if r.returncode != 0:
+                raise RuntimeError(r.stderr.decode())
+        out[f"{label}_s"] = min(times)
+    out["seqs_per_s"] = n / out["no_output_s"]
+    out["published_reference_s"] = 5.449
+    out["speedup_vs_published"] = 5.449 / out["no_output_s"]
+    for fn in os.listdir(tmp):
+        os.remove(os.path.join(tmp, fn))
+    os.rmdir(tmp)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +190,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-sets", type=int, default=1000)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--served-index", type=int, default=10_000_000)
     ap.add_argument("--served-queries", type=int, default=64)
     ap.add_argument("--served-rate", type=float, default=200.0)
@@ -240,6 +276,8 @@ def main():
     if not args.no_batched:
         result["batched"] = batched_vs_plain(eng, offsets, args.batched_sets, seed)
     eng.close()
+    if not args.no_e2e:
+        result["end_to_end_cli"] = end_to_end(chars, offsets, d, measure)
     if not args.no_served:
         result["served"] = served(args.served_index, args.served_queries, args.served_rate, 5)
     if not args.no_cpu_baseline:

@@ -61,6 +61,9 @@ def lib() -> ctypes.CDLL:
     L.xo_nn_brute.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.POINTER(i32p)]
     L.xo_nn_brute.restype = ctypes.c_int64
+    L.xo_neighbors_of.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.POINTER(i32p)]
+    L.xo_neighbors_of.restype = ctypes.c_int64
     L.xo_overlap_terms.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, i32p, i32p, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint64))]
     L.xo_overlap_terms.restype = ctypes.c_int64
@@ -203,3 +206,12 @@ def overlap(chars, offsets, freq, rep, d: int, measure: int = 0, which: int = 3)
     arr = np.ctypeslib.as_array(out, shape=(3 * m,)).copy() if m else np.zeros(0, np.uint64)
     lib().xo_free(ctypes.cast(out, ctypes.c_void_p))
     return arr.reshape(-1, 3)
+
+
+def neighbors_of(chars, offsets, row: int, d: int, measure: int = 0) -> np.ndarray:
+    """int64[m,2] (j, dist): every other row within distance d of `row`"""
+    chars, offsets = _seq_args(chars, offsets)
+    out = ctypes.POINTER(ctypes.c_int32)()
+    m = lib().xo_neighbors_of(chars.ctypes.data_as(ctypes.c_char_p), _p(offsets), len(offsets) - 1,
+                              row, d, measure, ctypes.byref(out))
+    return _take_i32(out, 2 * m).reshape(-1, 2).astype(np.int64)

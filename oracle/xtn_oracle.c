@@ -478,6 +478,34 @@ int64_t xo_nn_brute(const char* chars, const uint32_t* offsets, int n, int dista
 	return len;
 }
 
+/* every j != row with dist(row, j) <= distance, ascending j (the brute-force
+ * definition restricted to one row). *out receives 2 int32 per hit (j, dist). */
+int64_t xo_neighbors_of(const char* chars, const uint32_t* offsets, int n, int row, int distance,
+                        int measure, int32_t** out) {
+	int64_t cap = 64, len = 0;
+	int32_t* res = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)cap);
+	const int lr = (int)(offsets[row + 1] - offsets[row]);
+	for (int j = 0; j < n; j++) {
+		if (j == row)
+			continue;
+		int lj = (int)(offsets[j + 1] - offsets[j]);
+		if (measure == 0 && (lr - lj > distance || lj - lr > distance))
+			continue;
+		int d = xo_distance(chars + offsets[row], lr, chars + offsets[j], lj, measure);
+		if (d > distance)
+			continue;
+		if (len == cap) {
+			cap *= 2;
+			res = (int32_t*)realloc(res, sizeof(int32_t) * 2 * (size_t)cap);
+		}
+		res[2 * len] = j;
+		res[2 * len + 1] = d;
+		len++;
+	}
+	*out = res;
+	return len;
+}
+
 /* -------------------------------------------------------------- overlap -- */
 
 typedef struct {

@@ -77,6 +77,11 @@ int64_t xo_nn_port(const char* chars, const uint32_t* offsets, int n, int distan
 int64_t xo_nn_brute(const char* chars, const uint32_t* offsets, int n, int distance,
                     int measure, int32_t** out);
 
+/* the brute force for one row: every j != row within the distance,
+ * ascending; *out receives (j, dist) int32 pairs. */
+int64_t xo_neighbors_of(const char* chars, const uint32_t* offsets, int n, int row, int distance,
+                        int measure, int32_t** out);
+
 /* repertoire overlap, the reference algorithm restated. freq/rep are per input
  * row. *out receives 3 uint64 per entry (A, B, sum), sorted by (A, B). */
 int64_t xo_overlap_port(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,

@@ -216,3 +216,15 @@ def test_overlap_closed_form(seed):
     M = Rm.T.dot(A + np.identity(len(uniq), dtype=object)).dot(Rm)
     exp = [[a, b, int(M[a, b])] for a in range(R) for b in range(a, R) if M[a, b] != 0]
     assert got.tolist() == exp
+
+
+def test_neighbors_of_matches_brute():
+    rng = np.random.default_rng(42)
+    seqs = clustered_seqs(rng, 250, 2, 25, "ACDEF", edits=2)
+    chars, offsets = xtn.encode(seqs)
+    for d, measure in [(1, 0), (2, 0), (1, 1)]:
+        tri = oracle.nn_brute(chars, offsets, d, measure)
+        for row in (0, 7, 120, 249):
+            want = sorted([(int(b), int(c)) for a, b, c in tri if a == row] +
+                          [(int(a), int(c)) for a, b, c in tri if b == row])
+            assert oracle.neighbors_of(chars, offsets, row, d, measure).tolist() == [list(x) for x in want]

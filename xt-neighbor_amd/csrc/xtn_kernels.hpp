@@ -132,13 +132,40 @@ __device__ __forceinline__ void init_slot_masks(u128* hm, u128* lm, int t, int n
 	}
 }
 
+constexpr int PAIR_LUT = 153;    // C(18, 2) deletion pairs
+constexpr int TRIPLE_LUT = 816;  // C(18, 3) deletion triples
+
+// colex ranks do not depend on L: the first C(L, j) j-subsets of {0..17} in
+// colex order are exactly the j-subsets of {0..L-1}. The tables hold the
+// positions of rank q, largest first, one byte each.
+__device__ __forceinline__ void init_colex_luts(uint16_t* pairs, uint32_t* triples, int t, int nt) {
+	for (int q = t; q < PAIR_LUT; q += nt) {
+		int b = 1;
+		while ((b + 1) * b / 2 <= q)
+			++b;
+		pairs[q] = (uint16_t)((b << 8) | (q - b * (b - 1) / 2));
+	}
+	for (int q = t; q < TRIPLE_LUT; q += nt) {
+		int c = 2;
+		while ((c + 1) * c * (c - 1) / 6 <= q)
+			++c;
+		const int r = q - c * (c - 1) * (c - 2) / 6;
+		int b = 1;
+		while ((b + 1) * b / 2 <= r)
+			++b;
+		triples[q] = (uint32_t)((c << 16) | (b << 8) | (r - b * (b - 1) / 2));
+	}
+}
+
 // the k-th deletion variant of a prefix of length L with at most m deletions.
 // Ranks enumerate deletion sets by size (0, 1, 2, ...), each size in colex
 // order. The order is internal: records are sorted by key afterwards.
 __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
                                                const uint32_t* __restrict__ binom /*19x19*/,
                                                const u128* __restrict__ hm,
-                                               const u128* __restrict__ lm) {
+                                               const u128* __restrict__ lm,
+                                               const uint16_t* __restrict__ pair_lut,
+                                               const uint32_t* __restrict__ triple_lut) {
 	if (k == 0)
 		return p;
 	k -= 1;
@@ -148,18 +175,22 @@ __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
 	if (m >= 2) {
 		const uint32_t c2 = binom[L * 19 + 2];
 		if (k < c2) {
-			// largest b with b(b-1)/2 <= k
-			uint32_t b = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)k)) * 0.5f);
-			while (b * (b - 1) / 2 > k)
-				--b;
-			while ((b + 1) * b / 2 <= k)
-				++b;
-			const uint32_t a = k - b * (b - 1) / 2;
-			return del_slot_t(del_slot_t(p, (int)b, hm, lm), (int)a, hm, lm);
+			const uint32_t ab = pair_lut[k];
+			return del_slot_t(del_slot_t(p, (int)(ab >> 8), hm, lm), (int)(ab & 0xFF), hm, lm);
 		}
 		k -= c2;
 	}
-	for (int j = 3; j <= m; ++j) {
+	if (m >= 3) {
+		const uint32_t c3 = binom[L * 19 + 3];
+		if (k < c3) {
+			const uint32_t abc = triple_lut[k];
+			u128 v = del_slot_t(p, (int)(abc >> 16), hm, lm);
+			v = del_slot_t(v, (int)((abc >> 8) & 0xFF), hm, lm);
+			return del_slot_t(v, (int)(abc & 0xFF), hm, lm);
+		}
+		k -= c3;
+	}
+	for (int j = 4; j <= m; ++j) {
 		const uint32_t cj = binom[L * 19 + j];
 		if (k < cj) {
 			u128 v = p;
@@ -227,6 +258,8 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	__shared__ uint64_t s_voff[GEN_LDS_SEQS + 1];
 	__shared__ uint32_t s_binom[19 * 19];
 	__shared__ u128 s_hm[MAX_PREFIX], s_lm[MAX_PREFIX];
+	__shared__ uint16_t s_pair_lut[PAIR_LUT];
+	__shared__ uint32_t s_triple_lut[TRIPLE_LUT];
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
 	__shared__ uint32_t s_hist[MODE == GEN_HIST ? 4096 : 1];
@@ -249,6 +282,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		s_binom[i] = c;
 	}
 	init_slot_masks(s_hm, s_lm, t, GEN_NT);
+	init_colex_luts(s_pair_lut, s_triple_lut, t, GEN_NT);
 	if (MODE == GEN_DIRECT)
 		for (int i = t; i < RS_BINS; i += GEN_NT)
 			s_dig[i] = 0;
@@ -313,7 +347,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			const int L = (int)(hiw >> 56);
 			const int m = a.distance < L ? a.distance : L;
 			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
-			const u128 v = unrank_variant(p, L, m, k, s_binom, s_hm, s_lm);
+			const u128 v = unrank_variant(p, L, m, k, s_binom, s_hm, s_lm, s_pair_lut, s_triple_lut);
 			uint64_t hv = hash_variant(v);
 			if (a.set_of)  // variants of different query sets get different keys
 				hv ^= mix64(((uint64_t)a.set_of[sid] + 1ull) * 0x9E3779B97F4A7C15ull);
