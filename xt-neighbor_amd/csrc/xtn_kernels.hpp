@@ -564,20 +564,22 @@ struct BucketArgs {
 	unsigned long long* n_ovf;
 };
 
-constexpr int BK_BIN_BITS = 12;
+constexpr int BK_BIN_BITS = 11;
 constexpr int BK_BINS = 1 << BK_BIN_BITS;
 constexpr int BK_NT = 512;
-constexpr uint32_t BK_NONE = 0xFFFFFFFFu;
+constexpr int BK_PER = RS_TILE / BK_NT;  // records per thread
+constexpr uint16_t BK_NONE = 0xFFFF;
 
 // One workgroup per bucket of <= RS_TILE records: every record is pushed on
-// the LDS list of its bin (12 more hash bits, one atomicExch each), then each
+// the LDS list of its bin (11 more hash bits, one atomicExch each), then each
 // record walks the records pushed before it in its bin and emits the pairs
 // that share the full stored hash and hold two different ids. Every unordered
 // pair of a bin is visited exactly once; the emission order is irrelevant
-// because the pairs are sorted afterwards.
+// because the pairs are sorted afterwards. 48 KB of LDS: three workgroups
+// (24 waves) per CU.
 __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	__shared__ uint64_t s_keys[RS_TILE];
-	__shared__ uint32_t s_next[RS_TILE];
+	__shared__ uint16_t s_next[RS_TILE];
 	__shared__ uint32_t s_head[BK_BINS];
 	__shared__ uint32_t s_scan[BK_NT / WAVE + 1];
 	__shared__ unsigned long long s_base;
@@ -595,31 +597,51 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 		}
 		return;
 	}
-	const uint32_t bmask = (1u << a.bin_bits) - 1u;
+	uint64_t k[BK_PER];
+#pragma unroll
+	for (int i = 0; i < BK_PER; ++i) {
+		const int j = i * BK_NT + t;
+		k[i] = j < cnt ? a.rec[s + j] : 0ull;
+	}
 	for (int i = t; i < BK_BINS; i += BK_NT)
-		s_head[i] = BK_NONE;
-	for (int j = t; j < cnt; j += BK_NT)
-		s_keys[j] = a.rec[s + j];
+		s_head[i] = 0xFFFFFFFFu;
+#pragma unroll
+	for (int i = 0; i < BK_PER; ++i) {
+		const int j = i * BK_NT + t;
+		if (j < cnt)
+			s_keys[j] = k[i];
+	}
 	__syncthreads();
-	for (int j = t; j < cnt; j += BK_NT)
-		s_next[j] = atomicExch(&s_head[(uint32_t)(s_keys[j] >> a.bin_shift) & bmask], (uint32_t)j);
+	const uint32_t bmask = (1u << a.bin_bits) - 1u;
+#pragma unroll
+	for (int i = 0; i < BK_PER; ++i) {
+		const int j = i * BK_NT + t;
+		if (j < cnt)
+			s_next[j] = (uint16_t)atomicExch(&s_head[(uint32_t)(k[i] >> a.bin_shift) & bmask], (uint32_t)j);
+	}
 	__syncthreads();
 	uint32_t count = 0;
-	for (int j = t; j < cnt; j += BK_NT) {
-		const uint64_t kj = s_keys[j];
-		for (uint32_t k = s_next[j]; k != BK_NONE; k = s_next[k]) {
+#pragma unroll
+	for (int i = 0; i < BK_PER; ++i) {
+		const int j = i * BK_NT + t;
+		if (j >= cnt)
+			continue;
+		for (uint16_t n = s_next[j]; n != BK_NONE; n = s_next[n]) {
 			uint64_t p;
-			count += pair_of(kj, s_keys[k], a.idb, &p) ? 1u : 0u;
+			count += pair_of(k[i], s_keys[n], a.idb, &p) ? 1u : 0u;
 		}
 	}
 	unsigned long long slot = block_append<BK_NT>(&a.ctr->n_cand, count, s_scan, &s_base);
 	if (!count)
 		return;
-	for (int j = t; j < cnt; j += BK_NT) {
-		const uint64_t kj = s_keys[j];
-		for (uint32_t k = s_next[j]; k != BK_NONE; k = s_next[k]) {
+#pragma unroll
+	for (int i = 0; i < BK_PER; ++i) {
+		const int j = i * BK_NT + t;
+		if (j >= cnt)
+			continue;
+		for (uint16_t n = s_next[j]; n != BK_NONE; n = s_next[n]) {
 			uint64_t p;
-			if (pair_of(kj, s_keys[k], a.idb, &p)) {
+			if (pair_of(k[i], s_keys[n], a.idb, &p)) {
 				if ((int64_t)slot < a.pcap)
 					a.pairs[slot] = p;
 				++slot;
