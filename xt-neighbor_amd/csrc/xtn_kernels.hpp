@@ -319,7 +319,9 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	int idx = 0;
 	uint64_t kept[MODE == GEN_APPEND ? GEN_IT : 1];
 	uint32_t nkept = 0;
-#pragma unroll
+	// kept rolled: the body holds the whole unranking, sixteen copies of it
+	// would not fit the instruction cache
+#pragma unroll 1
 	for (int it = 0; it < GEN_IT; ++it) {
 		const int64_t r = r0 + (int64_t)it * GEN_NT + t;
 		const bool valid = r < r1;

@@ -85,6 +85,11 @@ struct xtn_handle {
 	std::vector<PendingEvent> spans;  // stage timing, always on
 	DBuf bstart, ovf, set_of;
 	int64_t last_pairs = 0;
+	// host copy of the prefix lengths, and per-row prefix sums of the
+	// variant counts for one distance: V of any row range without a readback
+	std::vector<uint8_t> prefix_len;
+	std::vector<uint64_t> vprefix;
+	int vprefix_d = 0;
 };
 
 namespace {
@@ -170,6 +175,19 @@ int grow_keep(xtn_handle* h, DBuf& b, size_t bytes, size_t keep) {
 	release(b);
 	b = nb;
 	return XTN_OK;
+}
+
+// sum_{k <= min(d, L)} C(L, k) for the 18-residue prefix of a len-residue
+// sequence (the count k_pack computes on the device)
+uint64_t host_variant_count(uint32_t len, int d) {
+	const int L = len < 18 ? (int)len : 18;
+	const int m = d < L ? d : L;
+	uint64_t c = 1, total = 1;
+	for (int j = 1; j <= m; ++j) {
+		c = c * (uint64_t)(L - j + 1) / (uint64_t)j;
+		total += c;
+	}
+	return total;
 }
 
 inline unsigned grid_for(int64_t n, int per_block) {
@@ -488,9 +506,23 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), nseq,
 	                        h->voff.as<uint64_t>() + nseq));
 	uint64_t V = 0;
-	HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + nseq, sizeof(uint64_t),
-	                          hipMemcpyDeviceToHost, s));
-	HIP_TRY(h, hipStreamSynchronize(s));
+	if (!seq_map && (int64_t)row_base + nseq <= (int64_t)h->prefix_len.size()) {
+		// the same closed form the device scan just summed
+		if (h->vprefix_d != d || h->vprefix.size() != h->prefix_len.size() + 1) {
+			uint64_t per[19];
+			for (int L = 0; L <= 18; ++L)
+				per[L] = host_variant_count((uint32_t)L, d);
+			h->vprefix.assign(h->prefix_len.size() + 1, 0);
+			for (size_t i = 0; i < h->prefix_len.size(); ++i)
+				h->vprefix[i + 1] = h->vprefix[i] + per[h->prefix_len[i]];
+			h->vprefix_d = d;
+		}
+		V = h->vprefix[row_base + nseq] - h->vprefix[row_base];
+	} else {
+		HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + nseq, sizeof(uint64_t),
+		                          hipMemcpyDeviceToHost, s));
+		HIP_TRY(h, hipStreamSynchronize(s));
+	}
 	st.n_variants = (int64_t)V;
 
 	// chunk plan: the records of one chunk (plus the sort buffer, ~32 B per
@@ -987,6 +1019,13 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 	h->n = n;
 	h->n_chars = total;
 	h->out_len = 0;
+	h->prefix_len.resize((size_t)n);
+	for (int32_t i = 0; i < n; ++i) {
+		const uint32_t len = offsets[i + 1] - offsets[i];
+		h->prefix_len[i] = (uint8_t)(len < 18 ? len : 18);
+	}
+	h->vprefix.clear();
+	h->vprefix_d = 0;
 	return XTN_OK;
 }
 
@@ -1257,17 +1296,6 @@ struct ServeResult {
 	int status = XTN_OK;
 	std::string err;
 };
-
-uint64_t host_variant_count(uint32_t len, int d) {
-	const int L = len < 18 ? (int)len : 18;
-	const int m = d < L ? d : L;
-	uint64_t c = 1, total = 1;
-	for (int j = 1; j <= m; ++j) {
-		c = c * (uint64_t)(L - j + 1) / (uint64_t)j;
-		total += c;
-	}
-	return total;
-}
 
 }  // namespace
 
