@@ -572,83 +572,126 @@ constexpr int BK_NT = 512;
 constexpr int BK_PER = RS_TILE / BK_NT;  // records per thread
 constexpr uint16_t BK_NONE = 0xFFFF;
 
-// One workgroup per bucket of <= RS_TILE records: every record is pushed on
-// the LDS list of its bin (11 more hash bits, one atomicExch each), then each
-// record walks the records pushed before it in its bin and emits the pairs
-// that share the full stored hash and hold two different ids. Every unordered
-// pair of a bin is visited exactly once; the emission order is irrelevant
-// because the pairs are sorted afterwards. 48 KB of LDS: three workgroups
-// (24 waves) per CU.
+constexpr int BK_STAGE = 384;  // pairs staged in LDS between flushes
+
+// Buckets of <= RS_TILE records, each handled by one workgroup in LDS: every
+// record is pushed on the list of its bin (11 more hash bits, one atomicExch
+// each), then each record walks the records pushed before it in its bin and
+// emits the pairs that share the full stored hash and hold two different
+// ids. Every unordered pair of a bin is visited exactly once; the emission
+// order is irrelevant because the pairs are sorted afterwards.
+// The workgroups are persistent: workgroup w takes buckets w, w + grid, ...,
+// loading the records of its next bucket into registers while it works on
+// the current one, and collects pairs in LDS (LDS atomics), appending them to
+// the global buffer with one atomic per flush. Every loop is bounded by the
+// bucket count; nothing waits on another workgroup. 52 KB of LDS: three
+// workgroups (24 waves) per CU.
 __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	__shared__ uint64_t s_keys[RS_TILE];
 	__shared__ uint16_t s_next[RS_TILE];
 	__shared__ uint32_t s_head[BK_BINS];
-	__shared__ uint32_t s_scan[BK_NT / WAVE + 1];
+	__shared__ uint64_t s_stage[BK_STAGE];
+	__shared__ uint32_t s_nstage;
 	__shared__ unsigned long long s_base;
 	const int t = threadIdx.x;
-	const int b = blockIdx.x;
-	const uint32_t s = a.bstart[b], e = a.bstart[b + 1];
-	const int cnt = (int)(e - s);
-	if (cnt < 2)
-		return;  // uniform over the workgroup
-	if (cnt > a.cap) {
-		if (t == 0) {
-			const unsigned long long slot = atomicAdd(a.n_ovf, 1ull);
-			if ((int64_t)slot < a.ovf_cap)
-				a.ovf[slot] = (uint32_t)b;
-		}
-		return;
-	}
-	uint64_t k[BK_PER];
-#pragma unroll
-	for (int i = 0; i < BK_PER; ++i) {
-		const int j = i * BK_NT + t;
-		k[i] = j < cnt ? a.rec[s + j] : 0ull;
-	}
-	for (int i = t; i < BK_BINS; i += BK_NT)
-		s_head[i] = 0xFFFFFFFFu;
-#pragma unroll
-	for (int i = 0; i < BK_PER; ++i) {
-		const int j = i * BK_NT + t;
-		if (j < cnt)
-			s_keys[j] = k[i];
-	}
-	__syncthreads();
 	const uint32_t bmask = (1u << a.bin_bits) - 1u;
+	if (t == 0)
+		s_nstage = 0;
+	uint64_t kn[BK_PER];
+	uint32_t s0 = 0, e0 = 0;
+	int b = blockIdx.x;
+	if (b < a.nb) {
+		s0 = a.bstart[b];
+		e0 = a.bstart[b + 1];
+		const int c = (int)(e0 - s0);
 #pragma unroll
-	for (int i = 0; i < BK_PER; ++i) {
-		const int j = i * BK_NT + t;
-		if (j < cnt)
-			s_next[j] = (uint16_t)atomicExch(&s_head[(uint32_t)(k[i] >> a.bin_shift) & bmask], (uint32_t)j);
-	}
-	__syncthreads();
-	uint32_t count = 0;
-#pragma unroll
-	for (int i = 0; i < BK_PER; ++i) {
-		const int j = i * BK_NT + t;
-		if (j >= cnt)
-			continue;
-		for (uint16_t n = s_next[j]; n != BK_NONE; n = s_next[n]) {
-			uint64_t p;
-			count += pair_of(k[i], s_keys[n], a.idb, &p) ? 1u : 0u;
+		for (int i = 0; i < BK_PER; ++i) {
+			const int j = i * BK_NT + t;
+			kn[i] = (c <= a.cap && j < c) ? a.rec[s0 + j] : 0ull;
 		}
 	}
-	unsigned long long slot = block_append<BK_NT>(&a.ctr->n_cand, count, s_scan, &s_base);
-	if (!count)
-		return;
+	__syncthreads();
+	while (b < a.nb) {  // uniform over the workgroup
+		uint64_t k[BK_PER];
 #pragma unroll
-	for (int i = 0; i < BK_PER; ++i) {
-		const int j = i * BK_NT + t;
-		if (j >= cnt)
-			continue;
-		for (uint16_t n = s_next[j]; n != BK_NONE; n = s_next[n]) {
-			uint64_t p;
-			if (pair_of(k[i], s_keys[n], a.idb, &p)) {
-				if ((int64_t)slot < a.pcap)
-					a.pairs[slot] = p;
-				++slot;
+		for (int i = 0; i < BK_PER; ++i)
+			k[i] = kn[i];
+		const int cnt = (int)(e0 - s0);
+		const int bn = b + (int)gridDim.x;
+		uint32_t s1 = 0, e1 = 0;
+		if (bn < a.nb) {  // next bucket's records in flight during this one
+			s1 = a.bstart[bn];
+			e1 = a.bstart[bn + 1];
+			const int c = (int)(e1 - s1);
+#pragma unroll
+			for (int i = 0; i < BK_PER; ++i) {
+				const int j = i * BK_NT + t;
+				kn[i] = (c <= a.cap && j < c) ? a.rec[s1 + j] : 0ull;
 			}
 		}
+		if (cnt > a.cap) {
+			if (t == 0) {
+				const unsigned long long slot = atomicAdd(a.n_ovf, 1ull);
+				if ((int64_t)slot < a.ovf_cap)
+					a.ovf[slot] = (uint32_t)b;
+			}
+		} else if (cnt >= 2) {
+			for (int i = t; i < BK_BINS; i += BK_NT)
+				s_head[i] = 0xFFFFFFFFu;
+#pragma unroll
+			for (int i = 0; i < BK_PER; ++i) {
+				const int j = i * BK_NT + t;
+				if (j < cnt)
+					s_keys[j] = k[i];
+			}
+			__syncthreads();
+#pragma unroll
+			for (int i = 0; i < BK_PER; ++i) {
+				const int j = i * BK_NT + t;
+				if (j < cnt)
+					s_next[j] = (uint16_t)atomicExch(&s_head[(uint32_t)(k[i] >> a.bin_shift) & bmask], (uint32_t)j);
+			}
+			__syncthreads();
+#pragma unroll
+			for (int i = 0; i < BK_PER; ++i) {
+				const int j = i * BK_NT + t;
+				if (j >= cnt)
+					continue;
+				for (uint16_t n = s_next[j]; n != BK_NONE; n = s_next[n]) {
+					uint64_t p;
+					if (!pair_of(k[i], s_keys[n], a.idb, &p))
+						continue;
+					const uint32_t slot = atomicAdd(&s_nstage, 1u);
+					if (slot < (uint32_t)BK_STAGE) {
+						s_stage[slot] = p;
+					} else {  // stage full: straight to the global buffer
+						const unsigned long long g = atomicAdd(&a.ctr->n_cand, 1ull);
+						if ((int64_t)g < a.pcap)
+							a.pairs[g] = p;
+					}
+				}
+			}
+			__syncthreads();
+		}
+		const uint32_t ns = s_nstage;
+		if (ns > (uint32_t)BK_STAGE / 2 || (bn >= a.nb && ns > 0)) {  // uniform
+			const uint32_t nf = ns < (uint32_t)BK_STAGE ? ns : (uint32_t)BK_STAGE;
+			__syncthreads();
+			if (t == 0)
+				s_base = atomicAdd(&a.ctr->n_cand, (unsigned long long)nf);
+			__syncthreads();
+			const unsigned long long base = s_base;
+			for (uint32_t i = t; i < nf; i += BK_NT)
+				if ((int64_t)(base + i) < a.pcap)
+					a.pairs[base + i] = s_stage[i];
+			__syncthreads();
+			if (t == 0)
+				s_nstage = 0;
+			__syncthreads();
+		}
+		b = bn;
+		s0 = s1;
+		e0 = e1;
 	}
 }
 

@@ -87,6 +87,7 @@ struct xtn_handle {
 	int64_t last_pairs = 0;
 	// host copy of the prefix lengths, and per-row prefix sums of the
 	// variant counts for one distance: V of any row range without a readback
+	int n_cu = 256;
 	std::vector<uint8_t> prefix_len;
 	std::vector<uint64_t> vprefix;
 	int vprefix_d = 0;
@@ -670,7 +671,9 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ba.pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
 			{
 				ProfScope ps(h, "bucket", (double)nrec * 8.0);
-				k_bucket<<<(unsigned)nb, BK_NT, 0, s>>>(ba);
+				// persistent workgroups, three per CU
+				const unsigned grid = (unsigned)std::min<int64_t>(nb, (int64_t)h->n_cu * 3);
+				k_bucket<<<grid, BK_NT, 0, s>>>(ba);
 				HIP_LAUNCH_CHECK(h);
 			}
 			TRY(read_ctrs(h, &rc));
@@ -952,6 +955,9 @@ int xtn_open(xtn_handle** out, const xtn_config* cfg) {
 		return XTN_ERR_HIP;
 	xtn_handle* h = new xtn_handle();
 	h->cfg = c;
+	int ncu = 0;
+	if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && ncu > 0)
+		h->n_cu = ncu;
 	memset(&h->stats, 0, sizeof(h->stats));
 	if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
 	    hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess) {
