@@ -135,26 +135,50 @@ __device__ __forceinline__ void init_slot_masks(u128* hm, u128* lm, int t, int n
 constexpr int PAIR_LUT = 153;    // C(18, 2) deletion pairs
 constexpr int TRIPLE_LUT = 816;  // C(18, 3) deletion triples
 
-// colex ranks do not depend on L: the first C(L, j) j-subsets of {0..17} in
-// colex order are exactly the j-subsets of {0..L-1}. The tables hold the
-// positions of rank q, largest first, one byte each.
-__device__ __forceinline__ void init_colex_luts(uint16_t* pairs, uint32_t* triples, int t, int nt) {
-	for (int q = t; q < PAIR_LUT; q += nt) {
-		int b = 1;
-		while ((b + 1) * b / 2 <= q)
-			++b;
-		pairs[q] = (uint16_t)((b << 8) | (q - b * (b - 1) / 2));
-	}
-	for (int q = t; q < TRIPLE_LUT; q += nt) {
-		int c = 2;
-		while ((c + 1) * c * (c - 1) / 6 <= q)
-			++c;
-		const int r = q - c * (c - 1) * (c - 2) / 6;
-		int b = 1;
-		while ((b + 1) * b / 2 <= r)
-			++b;
-		triples[q] = (uint32_t)((c << 16) | (b << 8) | (r - b * (b - 1) / 2));
-	}
+// Tables of the generator, fixed at compile time: binomials C(n, k) for
+// n, k <= 18, and the colex-ranked deletion pairs and triples. Colex ranks do
+// not depend on L: the first C(L, j) j-subsets of {0..17} in colex order are
+// exactly the j-subsets of {0..L-1}. A table entry holds the positions of its
+// rank, largest first, one byte each.
+struct GenTables {
+	uint32_t binom[19 * 19];
+	uint32_t triple[TRIPLE_LUT];
+	uint16_t pair[PAIR_LUT];
+	uint16_t pad;
+};
+
+constexpr GenTables make_gen_tables() {
+	GenTables g{};
+	for (int n = 0; n < 19; ++n)
+		for (int k = 0; k < 19; ++k) {
+			uint32_t c = 0;
+			if (k <= n) {
+				c = 1;
+				for (int j = 1; j <= k; ++j)
+					c = c * (uint32_t)(n - j + 1) / (uint32_t)j;
+			}
+			g.binom[n * 19 + k] = c;
+		}
+	int q = 0;
+	for (int hi = 1; hi < MAX_PREFIX; ++hi)
+		for (int lo = 0; lo < hi; ++lo)
+			g.pair[q++] = (uint16_t)((hi << 8) | lo);
+	q = 0;
+	for (int c = 2; c < MAX_PREFIX; ++c)
+		for (int b = 1; b < c; ++b)
+			for (int a = 0; a < b; ++a)
+				g.triple[q++] = (uint32_t)((c << 16) | (b << 8) | a);
+	return g;
+}
+
+__constant__ GenTables c_gen_tables = make_gen_tables();
+
+// LDS copy of the tables; every thread of the workgroup calls it
+__device__ __forceinline__ void load_gen_tables(GenTables* s, int t, int nt) {
+	const uint32_t* src = reinterpret_cast<const uint32_t*>(&c_gen_tables);
+	uint32_t* dst = reinterpret_cast<uint32_t*>(s);
+	for (int i = t; i < (int)(sizeof(GenTables) / 4); i += nt)
+		dst[i] = src[i];
 }
 
 // the k-th deletion variant of a prefix of length L with at most m deletions.
@@ -232,8 +256,8 @@ struct GenArgs {
 	unsigned long long* append;     // GEN_APPEND cursor
 	const uint32_t* set_of;         // nullable: query set of each id, salted into the key
 	uint32_t* digit_hist;           // GEN_DIRECT, nullable: per-tile counts of the
-	int64_t ntiles;                 // 8-bit digit at digit_shift, laid out as
-	int32_t digit_shift;            // k_radix_hist writes them
+	int64_t ntiles;                 // 8-bit digit at digit_shift, one row of
+	int32_t digit_shift;            // RS_BINS per tile as k_radix_hist writes them
 	int32_t pad2;
 };
 
@@ -256,10 +280,8 @@ __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n,
 template <int MODE>
 __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	__shared__ uint64_t s_voff[GEN_LDS_SEQS + 1];
-	__shared__ uint32_t s_binom[19 * 19];
+	__shared__ GenTables s_tab;
 	__shared__ u128 s_hm[MAX_PREFIX], s_lm[MAX_PREFIX];
-	__shared__ uint16_t s_pair_lut[PAIR_LUT];
-	__shared__ uint32_t s_triple_lut[TRIPLE_LUT];
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
 	__shared__ uint32_t s_hist[MODE == GEN_HIST ? 4096 : 1];
@@ -271,18 +293,8 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		return;
 	const int64_t r1 = r0 + GEN_TILE < a.V ? r0 + GEN_TILE : a.V;
 
-	for (int i = t; i < 19 * 19; i += GEN_NT) {
-		const int nn = i / 19, kk = i % 19;
-		uint32_t c = 0;
-		if (kk <= nn) {
-			c = 1;
-			for (int j = 1; j <= kk; ++j)
-				c = c * (uint32_t)(nn - j + 1) / (uint32_t)j;
-		}
-		s_binom[i] = c;
-	}
+	load_gen_tables(&s_tab, t, GEN_NT);
 	init_slot_masks(s_hm, s_lm, t, GEN_NT);
-	init_colex_luts(s_pair_lut, s_triple_lut, t, GEN_NT);
 	if (MODE == GEN_DIRECT)
 		for (int i = t; i < RS_BINS; i += GEN_NT)
 			s_dig[i] = 0;
@@ -349,7 +361,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			const int L = (int)(hiw >> 56);
 			const int m = a.distance < L ? a.distance : L;
 			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
-			const u128 v = unrank_variant(p, L, m, k, s_binom, s_hm, s_lm, s_pair_lut, s_triple_lut);
+			const u128 v = unrank_variant(p, L, m, k, s_tab.binom, s_hm, s_lm, s_tab.pair, s_tab.triple);
 			uint64_t hv = hash_variant(v);
 			if (a.set_of)  // variants of different query sets get different keys
 				hv ^= mix64(((uint64_t)a.set_of[sid] + 1ull) * 0x9E3779B97F4A7C15ull);
@@ -381,7 +393,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	if (MODE == GEN_DIRECT && a.digit_hist) {
 		__syncthreads();
 		for (int i = t; i < RS_BINS; i += GEN_NT)
-			a.digit_hist[(int64_t)i * a.ntiles + blockIdx.x] = s_dig[i];
+			a.digit_hist[(int64_t)blockIdx.x * RS_BINS + i] = s_dig[i];
 	}
 	if (MODE == GEN_HIST) {
 		__syncthreads();

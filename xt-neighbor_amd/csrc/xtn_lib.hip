@@ -315,6 +315,23 @@ int scan_excl(xtn_handle* h, const T* in, T* out, int64_t n, T* d_total) {
 	return XTN_OK;
 }
 
+// Per-tile digit counts (rows of RS_BINS) -> output bases, in place.
+int digit_scan(xtn_handle* h, uint32_t* hist, int64_t ntiles) {
+	hipStream_t s = h->stream;
+	const int64_t ngroups = (ntiles + DS_GROUP - 1) / DS_GROUP;
+	TRY(ensure(h, h->scan_tmp, sizeof(uint32_t) * (size_t)(RS_BINS * (ngroups + 1))));
+	uint32_t* gsum = h->scan_tmp.as<uint32_t>();
+	uint32_t* dtot = gsum + RS_BINS * ngroups;
+	ProfScope ps(h, "scan", 3.0 * 4.0 * RS_BINS * (double)ntiles);
+	k_dscan_group<<<(unsigned)ngroups, RS_BINS, 0, s>>>(hist, ntiles, gsum);
+	HIP_LAUNCH_CHECK(h);
+	k_dscan_cols<<<RS_BINS, SC_NT, 0, s>>>(gsum, ngroups, dtot);
+	HIP_LAUNCH_CHECK(h);
+	k_dscan_apply<<<(unsigned)ngroups, RS_BINS, 0, s>>>(hist, ntiles, gsum, dtot);
+	HIP_LAUNCH_CHECK(h);
+	return XTN_OK;
+}
+
 // Sorts n keys (and values when vals != nullptr) by bits [b0, b1) with one
 // stable 8-bit LSD pass per digit, ping-ponging between the pointer pairs; on
 // return `keys`/`vals` point at the buffers holding the result.
@@ -338,8 +355,7 @@ int radix_sort_ptr(xtn_handle* h, uint64_t*& keys, uint64_t*& keys_alt, uint32_t
 			k_radix_hist<<<(unsigned)ntiles, RS_NT, 0, s>>>(keys, n, shift, h->hist.as<uint32_t>(), ntiles);
 			HIP_LAUNCH_CHECK(h);
 		}
-		TRY(scan_excl<uint32_t>(h, h->hist.as<uint32_t>(), h->hist.as<uint32_t>(), RS_BINS * ntiles,
-		                        nullptr));
+		TRY(digit_scan(h, h->hist.as<uint32_t>(), ntiles));
 		{
 			ProfScope ps(h, scatter_name.c_str(), 2.0 * key_bytes);
 			if (kv)

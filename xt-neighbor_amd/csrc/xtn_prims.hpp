@@ -5,8 +5,13 @@
 //
 // Radix sort: reduce-then-scan per 8-bit digit pass.
 //   k_radix_hist    one 4096-key tile per workgroup, LDS digit histogram,
-//                   counts written digit-major so one exclusive scan yields
-//                   every (digit, tile) output base.
+//                   written as the tile's row of 256 counts (tile-major, so
+//                   every store is one contiguous 1 KB row).
+//   k_dscan_*       turn the rows into output bases in place: the base of
+//                   (tile, digit) counts every key of a smaller digit plus
+//                   the keys of that digit in earlier tiles. Group sums over
+//                   DS_GROUP tiles, one exclusive scan per digit column of
+//                   the group sums, then each group rewrites its rows.
 //   k_radix_scatter the tile is ranked stably inside each wave with eight
 //                   ballots (the lanes holding the same digit), wave-private
 //                   LDS counters carry the rank across the wave's 16 rounds,
@@ -219,7 +224,77 @@ __global__ __launch_bounds__(RS_NT) void k_radix_hist(const uint64_t* __restrict
 #pragma unroll
 	for (int ww = 0; ww < RS_WAVES; ++ww)
 		c += s_h[ww][d];
-	hist[(int64_t)d * ntiles + blockIdx.x] = c;
+	hist[(int64_t)blockIdx.x * RS_BINS + d] = c;
+}
+
+// ------------------------------------------------- digit-offset scan --
+
+constexpr int DS_GROUP = 32;  // tiles per group
+
+// gsum[g][d] = sum of hist[t][d] over the tiles t of group g
+__global__ __launch_bounds__(RS_BINS) void k_dscan_group(const uint32_t* __restrict__ hist,
+                                                         int64_t ntiles, uint32_t* __restrict__ gsum) {
+	const int d = threadIdx.x;
+	const int64_t t0 = (int64_t)blockIdx.x * DS_GROUP;
+	const int nt = ntiles - t0 < DS_GROUP ? (int)(ntiles - t0) : DS_GROUP;
+	uint32_t acc = 0;
+#pragma unroll 8
+	for (int i = 0; i < nt; ++i)
+		acc += hist[(t0 + i) * RS_BINS + d];
+	gsum[(int64_t)blockIdx.x * RS_BINS + d] = acc;
+}
+
+// one workgroup per digit d: exclusive scan of column d of gsum in place,
+// the column total into dtot[d]
+__global__ __launch_bounds__(SC_NT) void k_dscan_cols(uint32_t* __restrict__ gsum, int64_t ngroups,
+                                                      uint32_t* __restrict__ dtot) {
+	__shared__ uint32_t s_tmp[SC_NT / WAVE + 1];
+	const int d = blockIdx.x;
+	uint32_t carry = 0;
+	for (int64_t base = 0; base < ngroups; base += SC_TILE) {
+		uint32_t vals[SC_IT];
+		uint32_t sum = 0;
+#pragma unroll
+		for (int i = 0; i < SC_IT; ++i) {
+			const int64_t g = base + (int64_t)threadIdx.x * SC_IT + i;
+			vals[i] = g < ngroups ? gsum[g * RS_BINS + d] : 0u;
+			sum += vals[i];
+		}
+		uint32_t tot;
+		uint32_t run = carry + block_excl_scan<SC_NT>(sum, s_tmp, &tot);
+#pragma unroll
+		for (int i = 0; i < SC_IT; ++i) {
+			const int64_t g = base + (int64_t)threadIdx.x * SC_IT + i;
+			if (g < ngroups)
+				gsum[g * RS_BINS + d] = run;
+			run += vals[i];
+		}
+		carry += tot;
+	}
+	if (threadIdx.x == 0)
+		dtot[d] = carry;
+}
+
+// rows of group g rewritten in place as output bases
+__global__ __launch_bounds__(RS_BINS) void k_dscan_apply(uint32_t* __restrict__ hist, int64_t ntiles,
+                                                         const uint32_t* __restrict__ gsum,
+                                                         const uint32_t* __restrict__ dtot) {
+	__shared__ uint32_t s_tmp[RS_BINS / WAVE + 1];
+	const int d = threadIdx.x;
+	uint32_t run = block_excl_scan<RS_BINS>(dtot[d], s_tmp, (uint32_t*)nullptr) +
+	               gsum[(int64_t)blockIdx.x * RS_BINS + d];
+	const int64_t t0 = (int64_t)blockIdx.x * DS_GROUP;
+	const int nt = ntiles - t0 < DS_GROUP ? (int)(ntiles - t0) : DS_GROUP;
+	uint32_t v[DS_GROUP];
+#pragma unroll
+	for (int i = 0; i < DS_GROUP; ++i)
+		v[i] = i < nt ? hist[(t0 + i) * RS_BINS + d] : 0u;
+#pragma unroll
+	for (int i = 0; i < DS_GROUP; ++i)
+		if (i < nt) {
+			hist[(t0 + i) * RS_BINS + d] = run;
+			run += v[i];
+		}
 }
 
 // lanes of the wave whose digit equals this lane's (among valid lanes)
@@ -329,7 +404,7 @@ __global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restr
 		if (KV)
 			v[r] = valid ? vals_in[idx] : 0u;
 	}
-	s_gbase[t] = offs[(int64_t)t * ntiles + tile];
+	s_gbase[t] = offs[tile * RS_BINS + t];
 	uint32_t lp[RS_IT];
 	rank_tile(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 #pragma unroll
