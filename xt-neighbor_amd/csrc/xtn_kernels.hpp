@@ -862,43 +862,242 @@ __device__ __forceinline__ int seq_distance(const uint8_t* a, int la, const uint
 	}
 }
 
-// one lane per sorted candidate pair: duplicates of the previous pair are
-// skipped (DeviceSelect::Unique, xtn_inner.cu:138-142), the rest verified on
-// the full strings and flagged when distance <= d (kernel.cu:359-392)
-__global__ __launch_bounds__(256) void k_verify(const uint64_t* __restrict__ pairs, int64_t n,
-                                                int idb, const uint8_t* __restrict__ chars,
-                                                const uint32_t* __restrict__ offs,
-                                                const uint32_t* __restrict__ seq_map, uint32_t row_base,
-                                                const uint32_t* __restrict__ set_of, int distance,
-                                                int measure, uint32_t* __restrict__ flags,
-                                                int8_t* __restrict__ dist,
-                                                RunCtrs* __restrict__ ctr) {
-	const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	bool uniq = false;
-	uint32_t keep = 0;
-	int8_t dv = 0;
-	if (q < n) {
-		const uint64_t p = pairs[q];
-		uniq = q == 0 || pairs[q - 1] != p;
-		if (uniq) {
-			const uint64_t mask = (1ull << idb) - 1ull;
-			const uint32_t i = (uint32_t)(p >> idb), j = (uint32_t)(p & mask);
-			const uint32_t oi = seq_map ? seq_map[i] : row_base + i;
-			const uint32_t oj = seq_map ? seq_map[j] : row_base + j;
-			if (!set_of || set_of[i] == set_of[j]) {
-				const uint32_t si = offs[oi], sj = offs[oj];
-				const int li = (int)(offs[oi + 1] - si), lj = (int)(offs[oj + 1] - sj);
-				const int d = seq_distance(chars + si, li, chars + sj, lj, distance, measure);
-				keep = d <= distance ? 1u : 0u;
-				dv = (int8_t)d;
+// ---- register path for strings of at most REG_LEN residues (CDR3s) ----
+// Both strings are loaded once into registers, four residues per word; the
+// band DP is unrolled over the rows so every residue access is a constant
+// shift. Same results as lev_band / hamming on the same strings.
+constexpr int REG_LEN = 32;
+constexpr int REG_WORDS = REG_LEN / 4;
+
+// the residues p[0..len) (len <= REG_LEN) into w, byte j of the string in
+// byte j % 4 of w[j / 4]. Loads only the aligned words holding a byte of the
+// string: an aligned word never straddles a page, so no load leaves the
+// allocation. Bytes of w past len are unspecified.
+__device__ __forceinline__ void load_residues(const uint8_t* p, int len, uint32_t (&w)[REG_WORDS]) {
+	const uintptr_t addr = (uintptr_t)p;
+	const uint32_t* base = (const uint32_t*)(addr & ~(uintptr_t)3);
+	const uint32_t sh = (uint32_t)(addr & 3);
+	const int nw = len > 0 ? (int)((sh + (uint32_t)len - 1) >> 2) + 1 : 0;  // <= REG_WORDS + 1
+	uint32_t raw[REG_WORDS + 1];
+#pragma unroll
+	for (int k = 0; k <= REG_WORDS; ++k)
+		raw[k] = k < nw ? base[k] : 0u;
+#pragma unroll
+	for (int k = 0; k < REG_WORDS; ++k)
+		w[k] = __builtin_amdgcn_alignbyte(raw[k + 1], raw[k], sh);
+}
+
+__device__ __forceinline__ uint32_t residue(const uint32_t (&w)[REG_WORDS], int j) {
+	return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;  // j is a compile-time constant at every call
+}
+
+template <int D>
+__device__ __forceinline__ int lev_band_reg(const uint32_t (&A)[REG_WORDS], int la,
+                                            const uint32_t (&B)[REG_WORDS], int lb) {
+	constexpr int W = 2 * D + 1;
+	constexpr int INF = D + 1;
+	if (la - lb > D || lb - la > D)
+		return INF;
+	int row[W];
+#pragma unroll
+	for (int k = 0; k < W; ++k) {
+		const int x = k - D;
+		row[k] = (x >= 0 && x <= lb) ? (x < INF ? x : INF) : INF;
+	}
+#pragma unroll
+	for (int y = 1; y <= REG_LEN; ++y) {
+		if (y <= la) {  // rows past la: skipped by the whole wave once no lane needs them
+			const uint32_t ca = residue(A, y - 1);
+			int left = INF;
+#pragma unroll
+			for (int k = 0; k < W; ++k) {
+				const int x = y - D + k;
+				int val = INF;
+				if (x >= 0 && x <= REG_LEN && x <= lb) {
+					if (x == 0) {
+						val = y < INF ? y : INF;
+					} else {
+						const int diag = row[k] + (ca != residue(B, x - 1) ? 1 : 0);
+						const int up = (k + 1 < W ? row[k + 1] : INF) + 1;
+						val = diag < up ? diag : up;
+						val = left + 1 < val ? left + 1 : val;
+						val = val < INF ? val : INF;
+					}
+				}
+				row[k] = val;
+				left = val;
 			}
 		}
 	}
-	__shared__ unsigned long long s_tmp[256 / WAVE + 1];
-	block_add<256>(&ctr->n_unique, uniq ? 1ull : 0ull, s_tmp);
-	if (q < n) {
-		flags[q] = keep;
-		dist[q] = dv;
+	int r = INF;
+#pragma unroll
+	for (int k = 0; k < W; ++k)
+		if (k == lb - la + D)
+			r = row[k];
+	return r;
+}
+
+__device__ __forceinline__ int hamming_reg(const uint32_t (&A)[REG_WORDS], int la,
+                                           const uint32_t (&B)[REG_WORDS], int lb) {
+	if (la != lb)
+		return HAMMING_SENTINEL;
+	int d = 0;
+#pragma unroll
+	for (int j = 0; j < REG_LEN; ++j)
+		d += (j < la && residue(A, j) != residue(B, j)) ? 1 : 0;
+	return d;
+}
+
+// seq_distance with the register path when both strings fit it
+__device__ __forceinline__ int pair_distance(const uint8_t* a, int la, const uint8_t* b, int lb,
+                                             int distance, int measure) {
+	if (la > REG_LEN || lb > REG_LEN || (measure != 1 && distance > 8))
+		return seq_distance(a, la, b, lb, distance, measure);
+	uint32_t A[REG_WORDS], B[REG_WORDS];
+	load_residues(a, la, A);
+	load_residues(b, lb, B);
+	if (measure == 1)
+		return hamming_reg(A, la, B, lb);
+	switch (distance) {
+	case 1: return lev_band_reg<1>(A, la, B, lb);
+	case 2: return lev_band_reg<2>(A, la, B, lb);
+	case 3: return lev_band_reg<3>(A, la, B, lb);
+	case 4: return lev_band_reg<4>(A, la, B, lb);
+	case 5: return lev_band_reg<5>(A, la, B, lb);
+	case 6: return lev_band_reg<6>(A, la, B, lb);
+	case 7: return lev_band_reg<7>(A, la, B, lb);
+	default: return lev_band_reg<8>(A, la, B, lb);
+	}
+}
+
+
+// ------------------------------------------- unique + verify + select --
+
+// Sorted candidate pairs -> unique pairs -> distances -> output, every step
+// order-preserving, so the output comes out sorted. Tiles of SEL_TILE items,
+// SEL_IT consecutive items per thread for the order-preserving writes; the
+// counts of a step land in tile_cnt, whose in-place exclusive scan gives the
+// tile bases and (at tile_cnt[ntiles]) the total the next step runs over.
+// Verification runs over the compacted unique pairs only, so its lanes are
+// dense (a candidate pair repeats ~10 times in the sorted array).
+// Replaces the sort + unique (xtn_inner.cu:48-63) and the distance stage
+// with its output compaction (kernel.cu:235-300 cal_distance, xtn.cu:460-490).
+constexpr int SEL_NT = 256;
+constexpr int SEL_IT = 8;
+constexpr int SEL_TILE = SEL_NT * SEL_IT;
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_tmp) {
+	uint32_t total;
+	block_excl_scan<SEL_NT>(v, s_tmp, &total);
+	return total;
+}
+
+__global__ __launch_bounds__(SEL_NT) void k_uniq_count(const uint64_t* __restrict__ keys, int64_t n,
+                                                       uint32_t* __restrict__ tile_cnt) {
+	__shared__ uint32_t s_tmp[SEL_NT / WAVE + 1];
+	const int64_t base = (int64_t)blockIdx.x * SEL_TILE + (int64_t)threadIdx.x * SEL_IT;
+	uint32_t c = 0;
+#pragma unroll
+	for (int i = 0; i < SEL_IT; ++i) {
+		const int64_t q = base + i;
+		if (q < n && (q == 0 || keys[q - 1] != keys[q]))
+			++c;
+	}
+	const uint32_t tot = block_sum(c, s_tmp);
+	if (threadIdx.x == 0)
+		tile_cnt[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SEL_NT) void k_uniq_write(const uint64_t* __restrict__ keys, int64_t n,
+                                                       const uint32_t* __restrict__ tile_base,
+                                                       uint64_t* __restrict__ out) {
+	__shared__ uint32_t s_tmp[SEL_NT / WAVE + 1];
+	const int64_t base = (int64_t)blockIdx.x * SEL_TILE + (int64_t)threadIdx.x * SEL_IT;
+	uint32_t flags = 0, c = 0;
+#pragma unroll
+	for (int i = 0; i < SEL_IT; ++i) {
+		const int64_t q = base + i;
+		if (q < n && (q == 0 || keys[q - 1] != keys[q])) {
+			flags |= 1u << i;
+			++c;
+		}
+	}
+	uint64_t pos = tile_base[blockIdx.x] + block_excl_scan<SEL_NT>(c, s_tmp, (uint32_t*)nullptr);
+#pragma unroll
+	for (int i = 0; i < SEL_IT; ++i)
+		if (flags & (1u << i))
+			out[pos++] = keys[base + i];
+}
+
+// distances of the unique pairs, one per thread (a pair's gathers and DP are
+// latency-bound: the more pairs in flight the better); *n_dev of them (the
+// scan total of the previous step), the grid covers the largest possible
+// count. Tiles of SEL_NT pairs.
+__global__ __launch_bounds__(SEL_NT) void k_verify_count(const uint64_t* __restrict__ upairs,
+                                                         const uint32_t* __restrict__ n_dev, int idb,
+                                                         const uint8_t* __restrict__ chars,
+                                                         const uint32_t* __restrict__ offs,
+                                                         const uint32_t* __restrict__ seq_map,
+                                                         uint32_t row_base,
+                                                         const uint32_t* __restrict__ set_of,
+                                                         int distance, int measure,
+                                                         int8_t* __restrict__ dist,
+                                                         uint32_t* __restrict__ tile_cnt) {
+	__shared__ uint32_t s_tmp[SEL_NT / WAVE + 1];
+	const int64_t n = *n_dev;
+	const int64_t q = (int64_t)blockIdx.x * SEL_NT + threadIdx.x;
+	if ((int64_t)blockIdx.x * SEL_NT >= n) {  // uniform
+		if (threadIdx.x == 0)
+			tile_cnt[blockIdx.x] = 0;
+		return;
+	}
+	uint32_t c = 0;
+	{
+		const uint64_t mask = (1ull << idb) - 1ull;
+		if (q < n) {
+			const uint64_t p = upairs[q];
+			const uint32_t a = (uint32_t)(p >> idb), b = (uint32_t)(p & mask);
+			int8_t dv = -1;
+			if (!set_of || set_of[a] == set_of[b]) {
+				const uint32_t oa = seq_map ? seq_map[a] : row_base + a;
+				const uint32_t ob = seq_map ? seq_map[b] : row_base + b;
+				const uint32_t sa = offs[oa], sb = offs[ob];
+				const int d = pair_distance(chars + sa, (int)(offs[oa + 1] - sa), chars + sb,
+				                            (int)(offs[ob + 1] - sb), distance, measure);
+				if (d <= distance) {
+					dv = (int8_t)d;
+					++c;
+				}
+			}
+			dist[q] = dv;
+		}
+	}
+	const uint32_t tot = block_sum(c, s_tmp);
+	if (threadIdx.x == 0)
+		tile_cnt[blockIdx.x] = tot;
+}
+
+// the pairs within the distance, in order, as (i, j) and dist; tiles of
+// SEL_NT pairs as in k_verify_count
+__global__ __launch_bounds__(SEL_NT) void k_keep_write(const uint64_t* __restrict__ upairs,
+                                                       const uint32_t* __restrict__ n_dev, int idb,
+                                                       const int8_t* __restrict__ dist,
+                                                       const uint32_t* __restrict__ tile_base,
+                                                       int2* __restrict__ out_pairs,
+                                                       int8_t* __restrict__ out_dist) {
+	__shared__ uint32_t s_tmp[SEL_NT / WAVE + 1];
+	const int64_t n = *n_dev;
+	if ((int64_t)blockIdx.x * SEL_NT >= n)  // uniform
+		return;
+	const int64_t q = (int64_t)blockIdx.x * SEL_NT + threadIdx.x;
+	const int8_t dv = q < n ? dist[q] : (int8_t)-1;
+	const uint32_t keep = dv >= 0 ? 1u : 0u;
+	const uint64_t pos = tile_base[blockIdx.x] + block_excl_scan<SEL_NT>(keep, s_tmp, (uint32_t*)nullptr);
+	if (keep) {
+		const uint64_t p = upairs[q];
+		const uint64_t mask = (1ull << idb) - 1ull;
+		out_pairs[pos] = make_int2((int)(p >> idb), (int)(p & mask));
+		out_dist[pos] = dv;
 	}
 }
 

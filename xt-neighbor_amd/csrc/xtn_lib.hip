@@ -3,9 +3,12 @@
 //
 // Pipeline for one run (NN mode; overlap mode runs it on the unique strings):
 //   k_pack + scan          sequences -> packed prefixes, variant offsets (V)
-//   per key-space chunk:   k_gen (records) -> radix sort (run key bits)
-//                          -> k_runs -> k_pairs_small / k_pairs_big
-//   radix sort of pairs -> k_verify (unique + distance) -> scan -> k_emit_nn
+//   per key-space chunk:   k_gen (records, first digit histogram) -> two
+//                          radix passes over the bucket bits -> k_bucket
+//                          (candidate pairs from LDS bins; buckets too big
+//                          for LDS go through k_runs / k_pairs_*)
+//   radix sort of the candidate pairs -> unique (k_uniq_*) -> distances of
+//   the unique pairs (k_verify_count) -> the pairs within d (k_keep_write)
 // The reference runs the same stages as stream 1..4 of xtn_perform
 // (xtn.cu:267-509) with every intermediate staged through pinned host memory;
 // here everything between input upload and output download stays in HBM.
@@ -718,41 +721,51 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	st.n_candidates = n_pairs;
 	h->last_pairs = n_pairs;
 
-	// sort + unique + verify
+	// sort the candidates, keep the unique ones, verify those, keep the pairs
+	// within the distance; the last three steps preserve order
+	uint32_t nout = 0;
 	if (n_pairs > 0) {
 		StageScope sg(h, "verify");
 		TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)n_pairs));
 		TRY(radix_sort(h, h->pairs, h->pairs_alt, nullptr, nullptr, n_pairs, 0, 2 * idb, "sort_pairs"));
-		TRY(ensure(h, h->flags, sizeof(uint32_t) * (size_t)n_pairs));
-		TRY(ensure(h, h->pos, sizeof(uint32_t) * ((size_t)n_pairs + 1)));
+		const int64_t nt = (n_pairs + SEL_TILE - 1) / SEL_TILE;  // unique step tiles
+		const int64_t nv = (n_pairs + SEL_NT - 1) / SEL_NT;      // verify step tiles
+		TRY(ensure(h, h->pos, sizeof(uint32_t) * (size_t)(nt + 1 + nv + 1)));
 		TRY(ensure(h, h->dist, (size_t)n_pairs));
-		TRY(set_ctr(h, offsetof(RunCtrs, n_unique), 0ull));
+		uint32_t* ucnt = h->pos.as<uint32_t>();       // nt counts, total at [nt]
+		uint32_t* kcnt = ucnt + nt + 1;               // nv counts of kept pairs, total at [nv]
+		uint64_t* upairs = h->pairs_alt.as<uint64_t>();
 		{
-			ProfScope ps(h, "verify", (double)n_pairs * 13.0);
-			k_verify<<<grid_for(n_pairs, 256), 256, 0, s>>>(
-			    h->pairs.as<uint64_t>(), n_pairs, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
-			    seq_map, row_base, set_of, d, measure, h->flags.as<uint32_t>(), h->dist.as<int8_t>(),
-			    h->ctrs.as<RunCtrs>());
+			ProfScope ps(h, "unique", (double)n_pairs * 16.0);
+			k_uniq_count<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt);
+			HIP_LAUNCH_CHECK(h);
+			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(ucnt, nt, ucnt + nt);
+			HIP_LAUNCH_CHECK(h);
+			k_uniq_write<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt, upairs);
 			HIP_LAUNCH_CHECK(h);
 		}
-		TRY(scan_excl<uint32_t>(h, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), n_pairs,
-		                        h->pos.as<uint32_t>() + n_pairs));
-	}
-	uint32_t nout = 0;
-	if (n_pairs > 0) {
-		HIP_TRY(h, hipMemcpyAsync(&nout, h->pos.as<uint32_t>() + n_pairs, sizeof(uint32_t),
-		                          hipMemcpyDeviceToHost, s));
-		RunCtrs rc{};
-		TRY(read_ctrs(h, &rc));
-		st.n_unique = (int64_t)rc.n_unique;
+		{
+			ProfScope ps(h, "verify", 0.0);
+			k_verify_count<<<(unsigned)nv, SEL_NT, 0, s>>>(
+			    upairs, ucnt + nt, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), seq_map, row_base,
+			    set_of, d, measure, h->dist.as<int8_t>(), kcnt);
+			HIP_LAUNCH_CHECK(h);
+			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(kcnt, nv, kcnt + nv);
+			HIP_LAUNCH_CHECK(h);
+		}
+		uint32_t totals[2] = {0, 0};
+		HIP_TRY(h, hipMemcpyAsync(&totals[0], ucnt + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_TRY(h, hipMemcpyAsync(&totals[1], kcnt + nv, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+		HIP_TRY(h, hipStreamSynchronize(s));
+		st.n_unique = (int64_t)totals[0];
+		nout = totals[1];
 		TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
 		TRY(ensure(h, h->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
-		{
-			StageScope sg(h, "verify");
-			ProfScope ps(h, "emit", (double)n_pairs * 9.0 + (double)nout * 9.0);
-			k_emit_nn<<<grid_for(n_pairs, 256), 256, 0, s>>>(
-			    h->pairs.as<uint64_t>(), n_pairs, idb, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(),
-			    h->dist.as<int8_t>(), h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
+		if (nout > 0) {
+			const unsigned grid = (unsigned)((totals[0] + SEL_NT - 1) / SEL_NT);
+			ProfScope ps(h, "emit", (double)totals[0] * 9.0 + (double)nout * 9.0);
+			k_keep_write<<<grid, SEL_NT, 0, s>>>(upairs, ucnt + nt, idb, h->dist.as<int8_t>(), kcnt,
+			                                     h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
 			HIP_LAUNCH_CHECK(h);
 		}
 	}

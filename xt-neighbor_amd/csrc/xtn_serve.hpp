@@ -89,8 +89,8 @@ __global__ __launch_bounds__(256) void k_verify_cross(const uint64_t* __restrict
 	if (q == 0 || pairs[q - 1] != p) {
 		const uint32_t a = (uint32_t)(p >> idb), b = (uint32_t)(p & ((1ull << idb) - 1ull));
 		const uint32_t sa = qoffs[a], sb = ioffs[b];
-		const int d = seq_distance(qchars + sa, (int)(qoffs[a + 1] - sa), ichars + sb,
-		                           (int)(ioffs[b + 1] - sb), distance, measure);
+		const int d = pair_distance(qchars + sa, (int)(qoffs[a + 1] - sa), ichars + sb,
+		                            (int)(ioffs[b + 1] - sb), distance, measure);
 		keep = d <= distance ? 1u : 0u;
 		dv = (int8_t)d;
 	}
