@@ -242,7 +242,7 @@ def main():
         eng.nn_resident()
     hip_sync()
     profiled_ms = (time.perf_counter() - tp0) / args.steps * 1e3
-    kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "bucket", "runs", "pairs_small", "pairs_big",
+    kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "region", "bucket", "runs", "pairs_small", "pairs_big",
                "sort_seg_hist", "sort_seg_scatter", "sort_pairs_hist", "sort_pairs_scatter", "unique", "verify",
                "emit", "pack", "scan", "gen_count"]
     prof = {k: eng.kernel_profile(k) for k in kernels}

@@ -562,9 +562,121 @@ __global__ __launch_bounds__(256) void k_bucket_bounds(const uint64_t* __restric
 	bstart[b] = (uint32_t)lo;
 }
 
+// ------------------------------------------- region (second) pass --
+//
+// Buckets of the B > 8 bucket bits are laid out as fixed regions of
+// `stride` slots: bucket b owns [b * stride, b * stride + count[b]). The
+// records come from an exact (stable) pass over the top 8 bucket bits, so a
+// coarse bucket c (those 8 bits) is a contiguous range; tiles are cut inside
+// coarse buckets, every record of a tile shares c, and a tile ranks its
+// records by the remaining bucket bits in LDS and claims each run with one
+// atomic on the bucket's cursor (16 records per atomic on average). Slots past
+// the stride are not written: the cursor still counts them, which marks the
+// bucket as overflowing (more than the LDS bucket kernel holds), and the
+// chunk is then redone on the exact two-pass sort.
+
+// ctl[0..256] = first record of each coarse bucket (cstart, cstart[256] = n),
+// ctl[257..513] = first tile of each coarse bucket (tstart[256] = tiles)
+constexpr int CTL_WORDS = 2 * (RS_BINS + 1);
+
+__global__ __launch_bounds__(RS_BINS) void k_coarse_tiles(const uint32_t* __restrict__ dtot,
+                                                          uint32_t* __restrict__ ctl) {
+	__shared__ uint32_t s_tmp[RS_BINS / WAVE + 1];
+	const int c = threadIdx.x;
+	const uint32_t cnt = dtot[c];
+	uint32_t n;
+	const uint32_t cs = block_excl_scan<RS_BINS>(cnt, s_tmp, &n);
+	const uint32_t tiles = (cnt + RS_TILE - 1) / RS_TILE;
+	uint32_t nt;
+	const uint32_t ts = block_excl_scan<RS_BINS>(tiles, s_tmp, &nt);
+	ctl[c] = cs;
+	ctl[RS_BINS + 1 + c] = ts;
+	if (c == 0) {
+		ctl[RS_BINS] = n;
+		ctl[2 * RS_BINS + 1] = nt;
+	}
+}
+
+struct RegionArgs {
+	const uint64_t* in;     // sorted by the coarse digit
+	uint64_t* out;          // the regions
+	const uint32_t* ctl;    // k_coarse_tiles
+	uint32_t* count;        // 2^B cursors, zeroed
+	int64_t stride;         // slots per region
+	int32_t shift;          // lowest bucket bit: the bucket is bits [shift, shift + B)
+	int32_t fine_bits;      // B - 8
+};
+
+__global__ __launch_bounds__(RS_NT) void k_region_scatter(RegionArgs a) {
+	__shared__ uint64_t s_keys[RS_TILE];
+	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+	__shared__ uint32_t s_dpre[RS_BINS + 1];
+	__shared__ uint32_t s_gbase[RS_BINS];
+	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+	__shared__ uint32_t s_ts[RS_BINS + 1];
+
+	const int t = threadIdx.x;
+	const int lane = lane_id();
+	const int w = t / WAVE;
+	for (int i = t; i <= RS_BINS; i += RS_NT)
+		s_ts[i] = a.ctl[RS_BINS + 1 + i];
+	__syncthreads();
+	const uint32_t ntiles = s_ts[RS_BINS];
+	const uint32_t tile = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
+	if (tile >= ntiles)  // uniform
+		return;
+	int lo = 0, hi = RS_BINS - 1;  // the coarse bucket: largest c with s_ts[c] <= tile
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (s_ts[mid] <= tile)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const uint32_t c = (uint32_t)lo;
+	const int64_t cbeg = a.ctl[c], cend = a.ctl[c + 1];
+	const int64_t base = cbeg + (int64_t)(tile - s_ts[c]) * RS_TILE;
+	const int tile_n = (int)(cend - base < RS_TILE ? cend - base : RS_TILE);  // >= 1
+
+	const int64_t seg = base + (int64_t)w * RS_SEG;
+	const int wrem = tile_n - w * RS_SEG;
+	const int w_valid = wrem <= 0 ? 0 : (wrem < RS_SEG ? wrem : RS_SEG);
+	uint64_t k[RS_IT];
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r) {
+		const bool valid = r * WAVE + lane < w_valid;
+		k[r] = valid ? a.in[seg + (int64_t)r * WAVE + lane] : 0ull;
+	}
+	uint32_t lp[RS_IT];
+	rank_tile(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r)
+		if (lp[r] != 0xFFFFFFFFu)
+			s_keys[lp[r]] = k[r];
+	if (t == 0)
+		s_dpre[RS_BINS] = (uint32_t)tile_n;
+	__syncthreads();
+	// one cursor claim per digit run of the tile
+	const uint32_t fmask = (1u << a.fine_bits) - 1u;
+	{
+		const uint32_t cnt = s_dpre[t + 1] - s_dpre[t];
+		s_gbase[t] = cnt ? atomicAdd(&a.count[(c << a.fine_bits) | ((uint32_t)t & fmask)], cnt) : 0u;
+	}
+	__syncthreads();
+	for (int i = t; i < tile_n; i += RS_NT) {
+		const uint64_t key = s_keys[i];
+		const uint32_t d = (uint32_t)(key >> a.shift) & 0xFFu;
+		const uint32_t slot = s_gbase[d] + ((uint32_t)i - s_dpre[d]);
+		if ((int64_t)slot < a.stride)
+			a.out[(int64_t)((c << a.fine_bits) | (d & fmask)) * a.stride + slot] = key;
+	}
+}
+
 struct BucketArgs {
 	const uint64_t* rec;
-	const uint32_t* bstart;
+	const uint32_t* bstart;  // sorted layout: bucket b is [bstart[b], bstart[b + 1])
+	const uint32_t* bcount;  // region layout (non-null): [b * bstride, + bcount[b])
+	int64_t bstride;
 	int32_t nb;
 	int32_t bin_shift;  // the bucket is binned by bits [bin_shift, bin_shift + bin_bits)
 	int32_t bin_bits;   // <= BK_BIN_BITS
@@ -577,6 +689,17 @@ struct BucketArgs {
 	uint32_t* ovf;      // overflowing bucket ids
 	unsigned long long* n_ovf;
 };
+
+// records of bucket b: [*s, *e); e - s > cap marks an overflowing bucket
+__device__ __forceinline__ void bucket_range(const BucketArgs& a, int b, uint32_t* s, uint32_t* e) {
+	if (a.bcount) {
+		*s = (uint32_t)((int64_t)b * a.bstride);
+		*e = *s + a.bcount[b];
+	} else {
+		*s = a.bstart[b];
+		*e = a.bstart[b + 1];
+	}
+}
 
 constexpr int BK_BIN_BITS = 11;
 constexpr int BK_BINS = 1 << BK_BIN_BITS;
@@ -613,8 +736,7 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	uint32_t s0 = 0, e0 = 0;
 	int b = blockIdx.x;
 	if (b < a.nb) {
-		s0 = a.bstart[b];
-		e0 = a.bstart[b + 1];
+		bucket_range(a, b, &s0, &e0);
 		const int c = (int)(e0 - s0);
 #pragma unroll
 		for (int i = 0; i < BK_PER; ++i) {
@@ -632,8 +754,7 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 		const int bn = b + (int)gridDim.x;
 		uint32_t s1 = 0, e1 = 0;
 		if (bn < a.nb) {  // next bucket's records in flight during this one
-			s1 = a.bstart[bn];
-			e1 = a.bstart[bn + 1];
+			bucket_range(a, bn, &s1, &e1);
 			const int c = (int)(e1 - s1);
 #pragma unroll
 			for (int i = 0; i < BK_PER; ++i) {

@@ -73,6 +73,7 @@ struct xtn_handle {
 	int64_t n_chars = 0;
 
 	// workspaces (grow-only)
+	DBuf bcount, ctl;  // region layout: bucket cursors, coarse bucket/tile table
 	DBuf pk, voff, recs, recs_alt, hist, scan_tmp, runs_small, runs_big, big_pre, pairs, pairs_alt,
 	    flags, pos, dist, out_pairs, out_dist, ctrs, part_hist, scalar;
 	// overlap-mode workspaces
@@ -319,7 +320,7 @@ int scan_excl(xtn_handle* h, const T* in, T* out, int64_t n, T* d_total) {
 }
 
 // Per-tile digit counts (rows of RS_BINS) -> output bases, in place.
-int digit_scan(xtn_handle* h, uint32_t* hist, int64_t ntiles) {
+int digit_scan(xtn_handle* h, uint32_t* hist, int64_t ntiles, uint32_t** dtot_out = nullptr) {
 	hipStream_t s = h->stream;
 	const int64_t ngroups = (ntiles + DS_GROUP - 1) / DS_GROUP;
 	TRY(ensure(h, h->scan_tmp, sizeof(uint32_t) * (size_t)(RS_BINS * (ngroups + 1))));
@@ -332,6 +333,8 @@ int digit_scan(xtn_handle* h, uint32_t* hist, int64_t ntiles) {
 	HIP_LAUNCH_CHECK(h);
 	k_dscan_apply<<<(unsigned)ngroups, RS_BINS, 0, s>>>(hist, ntiles, gsum, dtot);
 	HIP_LAUNCH_CHECK(h);
+	if (dtot_out)  // the digit totals, valid until the next scan
+		*dtot_out = dtot;
 	return XTN_OK;
 }
 
@@ -545,14 +548,14 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	}
 	st.n_variants = (int64_t)V;
 
-	// chunk plan: the records of one chunk (plus the sort buffer, ~32 B per
-	// record in all) must fit the HBM budget, and its buckets must stay near
-	// BUCKET_TARGET records
+	// chunk plan: the records of one chunk (plus the sort buffer and the
+	// bucket regions, ~40 B per record in all) must fit the HBM budget, and
+	// its buckets must stay near BUCKET_TARGET records
 	int64_t budget = h->cfg.max_records_per_chunk;
 	if (budget <= 0) {
 		size_t fr = 0, tot = 0;
 		HIP_TRY(h, hipMemGetInfo(&fr, &tot));
-		budget = (int64_t)((double)fr * 0.6 / 32.0);
+		budget = (int64_t)((double)fr * 0.6 / 40.0);
 		budget = std::min<int64_t>(budget, (int64_t)65536 * BUCKET_TARGET);
 	}
 	budget = std::max<int64_t>(budget, 1 << 16);
@@ -610,6 +613,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	TRY(ensure(h, h->runs_small, sizeof(Run) * (size_t)(cap_recs / 2 + 1)));
 	TRY(ensure(h, h->runs_big, sizeof(Run) * (size_t)(cap_recs / (SMALL_RUN_MAX + 1) + 1)));
 	TRY(ensure(h, h->bstart, sizeof(uint32_t) * (65536 + 1)));
+	TRY(ensure(h, h->bcount, sizeof(uint32_t) * 65536));
+	TRY(ensure(h, h->ctl, sizeof(uint32_t) * CTL_WORDS));
 	TRY(ensure(h, h->ovf, sizeof(uint32_t) * 65536));
 	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, RS_TILE) : RS_TILE;
 
@@ -633,14 +638,21 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			++B;
 		B = std::min(B, K);
 		const int SB = (B + 7) / 8 * 8;
+		// more than 8 bucket bits: an exact pass over the top 8, then the
+		// region scatter (fixed regions of bucket_cap slots in recs)
+		const bool region = B > 8;
+		if (region) {
+			TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)((int64_t)bucket_cap << B)));
+			ga.out = h->recs.as<uint64_t>();
+		}
 		// the single-chunk generator also counts the first sort digit per tile
 		const bool fused_hist = nparts == 1 && B > 0;
 		const int64_t rtiles = (nrec + RS_TILE - 1) / RS_TILE;
+		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
 		if (fused_hist) {
-			TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
 			ga.digit_hist = h->hist.as<uint32_t>();
 			ga.ntiles = rtiles;
-			ga.digit_shift = hi - SB;
+			ga.digit_shift = region ? hi - 8 : hi - SB;
 		} else {
 			ga.digit_hist = nullptr;
 		}
@@ -659,17 +671,10 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				HIP_LAUNCH_CHECK(h);
 			}
 		}
-		if (B > 0) {
-			StageScope sg(h, "sort");
-			TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, hi - SB, hi, "sort_rec",
-			               fused_hist));
-		}
 		const int nb = 1 << B;
 		BucketArgs ba{};
-		ba.rec = h->recs.as<uint64_t>();
-		ba.bstart = h->bstart.as<uint32_t>();
 		ba.nb = nb;
-		// bins: the (up to) 12 stored hash bits right below the bucket bits
+		// bins: the (up to) 11 stored hash bits right below the bucket bits
 		ba.bin_bits = std::min(BK_BIN_BITS, hi - B - idb);
 		ba.bin_shift = hi - B - ba.bin_bits;
 		ba.idb = idb;
@@ -678,32 +683,102 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		ba.ctr = h->ctrs.as<RunCtrs>();
 		ba.ovf = h->ovf.as<uint32_t>();
 		ba.n_ovf = ctr_field(h, offsetof(RunCtrs, append));
+		const int64_t n_pairs_chunk0 = n_pairs;
 		RunCtrs rc{};
-		for (int attempt = 0; attempt < 2; ++attempt) {
-			StageScope sg(h, "pairs");
-			k_bucket_bounds<<<grid_for(nb + 1, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), nrec, B > 0 ? hi - B : 0,
-			                                                       (uint32_t)(nb - 1), nb, h->bstart.as<uint32_t>());
-			HIP_LAUNCH_CHECK(h);
-			TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
-			TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)n_pairs));
-			ba.pairs = h->pairs.as<uint64_t>();
-			ba.pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
-			{
-				ProfScope ps(h, "bucket", (double)nrec * 8.0);
-				// persistent workgroups, three per CU
-				const unsigned grid = (unsigned)std::min<int64_t>(nb, (int64_t)h->n_cu * 3);
-				k_bucket<<<grid, BK_NT, 0, s>>>(ba);
+		// bucket kernel over the layout ba names, the pair buffer regrown once
+		// if the candidates did not fit
+		auto run_buckets = [&]() -> int {
+			for (int attempt = 0; attempt < 2; ++attempt) {
+				StageScope sg(h, "pairs");
+				if (!ba.bcount) {
+					k_bucket_bounds<<<grid_for(nb + 1, 256), 256, 0, s>>>(
+					    ba.rec, nrec, B > 0 ? hi - B : 0, (uint32_t)(nb - 1), nb, h->bstart.as<uint32_t>());
+					HIP_LAUNCH_CHECK(h);
+				}
+				TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
+				TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)n_pairs_chunk0));
+				ba.pairs = h->pairs.as<uint64_t>();
+				ba.pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
+				{
+					ProfScope ps(h, "bucket", (double)nrec * 8.0);
+					// persistent workgroups, three per CU
+					const unsigned grid = (unsigned)std::min<int64_t>(nb, (int64_t)h->n_cu * 3);
+					k_bucket<<<grid, BK_NT, 0, s>>>(ba);
+					HIP_LAUNCH_CHECK(h);
+				}
+				TRY(read_ctrs(h, &rc));
+				if ((int64_t)rc.n_cand <= ba.pcap)
+					return XTN_OK;
+				if (attempt == 1)
+					return fail(h, XTN_ERR_HIP, "pair buffer overflow after regrow");
+				if (rc.n_cand > (unsigned long long)INT32_MAX)
+					return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31 (%llu)",
+					            (unsigned long long)rc.n_cand);
+				TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)(rc.n_cand + rc.n_cand / 4),
+				              sizeof(uint64_t) * (size_t)n_pairs_chunk0));
+			}
+			return XTN_OK;
+		};
+
+		bool sorted_layout = true;
+		if (region) {
+			// exact pass over the top 8 bucket bits, then the region scatter
+			// over the rest; buckets are read from their regions
+			StageScope sg(h, "sort");
+			uint32_t* hist = h->hist.as<uint32_t>();
+			if (!fused_hist) {
+				ProfScope ps(h, "sort_rec_hist", (double)nrec * 8.0);
+				k_radix_hist<<<(unsigned)rtiles, RS_NT, 0, s>>>(h->recs.as<uint64_t>(), nrec, hi - 8, hist, rtiles);
 				HIP_LAUNCH_CHECK(h);
 			}
-			TRY(read_ctrs(h, &rc));
-			if ((int64_t)rc.n_cand <= ba.pcap)
-				break;
-			if (attempt == 1)
-				return fail(h, XTN_ERR_HIP, "pair buffer overflow after regrow");
-			if (rc.n_cand > (unsigned long long)INT32_MAX)
-				return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31 (%llu)", (unsigned long long)rc.n_cand);
-			TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)(rc.n_cand + rc.n_cand / 4),
-			              sizeof(uint64_t) * (size_t)n_pairs));
+			uint32_t* dtot = nullptr;
+			TRY(digit_scan(h, hist, rtiles, &dtot));
+			k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
+			HIP_LAUNCH_CHECK(h);
+			{
+				ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)nrec);
+				k_radix_scatter<false><<<(unsigned)rtiles, RS_NT, 0, s>>>(
+				    h->recs.as<uint64_t>(), nullptr, h->recs_alt.as<uint64_t>(), nullptr, nrec, hi - 8, hist, rtiles);
+				HIP_LAUNCH_CHECK(h);
+			}
+			k_fill_u64<<<grid_for(nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), nb / 2, 0ull);
+			HIP_LAUNCH_CHECK(h);
+			RegionArgs ra{};
+			ra.in = h->recs_alt.as<uint64_t>();
+			ra.out = h->recs.as<uint64_t>();
+			ra.ctl = h->ctl.as<uint32_t>();
+			ra.count = h->bcount.as<uint32_t>();
+			ra.stride = bucket_cap;
+			ra.shift = hi - B;
+			ra.fine_bits = B - 8;
+			{
+				ProfScope ps(h, "region", 16.0 * (double)nrec);
+				k_region_scatter<<<(unsigned)(rtiles + RS_BINS), RS_NT, 0, s>>>(ra);
+				HIP_LAUNCH_CHECK(h);
+			}
+			ba.rec = h->recs.as<uint64_t>();
+			ba.bcount = h->bcount.as<uint32_t>();
+			ba.bstride = bucket_cap;
+			ba.bstart = nullptr;
+			TRY(run_buckets());
+			if (rc.append > 0) {
+				// a bucket outgrew its region: redo the chunk on the exact
+				// two-pass sort of the coarse-sorted records
+				std::swap(h->recs, h->recs_alt);
+				ba.bcount = nullptr;
+			} else {
+				sorted_layout = false;
+			}
+		}
+		if (sorted_layout) {
+			if (B > 0) {
+				StageScope sg(h, "sort");
+				TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, hi - SB, hi, "sort_rec",
+				               fused_hist && !region));
+			}
+			ba.rec = h->recs.as<uint64_t>();
+			ba.bstart = h->bstart.as<uint32_t>();
+			TRY(run_buckets());
 		}
 		n_pairs = (int64_t)rc.n_cand;
 		const int64_t n_ovf = (int64_t)rc.append;
@@ -1002,7 +1077,7 @@ void xtn_close(xtn_handle* h) {
 		return;
 	(void)hipSetDevice(h->cfg.device);
 	(void)hipStreamSynchronize(h->stream);
-	DBuf* bufs[] = {&h->chars, &h->offs, &h->pk, &h->voff, &h->recs, &h->recs_alt, &h->hist,
+	DBuf* bufs[] = {&h->bcount, &h->ctl, &h->chars, &h->offs, &h->pk, &h->voff, &h->recs, &h->recs_alt, &h->hist,
 	                &h->scan_tmp, &h->runs_small, &h->runs_big, &h->big_pre, &h->pairs,
 	                &h->pairs_alt, &h->flags, &h->pos, &h->dist, &h->out_pairs, &h->out_dist,
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
