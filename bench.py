@@ -179,6 +179,18 @@ def end_to_end(chars, offsets, d, measure):
     return out
 
 
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs summarised in profiles/ (calibrated on a copy kernel), or None"""
+    path = os.path.join(REPO, "profiles", f"{workload}_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return k["traffic_bytes"] if k else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,7 +291,8 @@ def main():
         "p50_ms": statistics.median(lat),
         "p99_ms": float(np.percentile(lat, 99)),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, dom)},
         "kernels": per_kernel,
         "reference_published": {"seqs_per_s": 183500, "note": "1M Emerson CDR3, d=2, 5.449 s end to "
                                 "end incl. process start and parsing, GPU unrecorded "

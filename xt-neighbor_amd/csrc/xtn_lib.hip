@@ -659,7 +659,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		{
 			StageScope sg(h, "generate");
 			if (nparts == 1) {
-				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0);
+				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (fused_hist ? 1024.0 * rtiles : 0.0));
 				k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
 				HIP_LAUNCH_CHECK(h);
 			} else {
