@@ -123,10 +123,18 @@ __device__ __forceinline__ u128 del_slot_t(u128 p, int pos, const u128* __restri
 	return (p & hm[pos]) | ((p & lm[pos]) << 5);
 }
 
+// slot NO_DEL (= MAX_PREFIX) is a no-op: hm keeps every slot, lm none
+constexpr int NO_DEL = MAX_PREFIX;
+
 __device__ __forceinline__ void init_slot_masks(u128* hm, u128* lm, int t, int nt) {
-	for (int pos = t; pos < MAX_PREFIX; pos += nt) {
-		const int low_bits = 5 * (MAX_PREFIX - 1 - pos);
+	for (int pos = t; pos <= MAX_PREFIX; pos += nt) {
 		const u128 one = 1;
+		if (pos == NO_DEL) {
+			hm[pos] = ~(u128)0;
+			lm[pos] = 0;
+			continue;
+		}
+		const int low_bits = 5 * (MAX_PREFIX - 1 - pos);
 		lm[pos] = (one << low_bits) - 1;
 		hm[pos] = ~((one << (low_bits + 5)) - 1);
 	}
@@ -172,6 +180,46 @@ constexpr GenTables make_gen_tables() {
 }
 
 __constant__ GenTables c_gen_tables = make_gen_tables();
+
+// Deletion sets of at most d <= 3 positions ordered by their largest
+// position: the sets inside a prefix of length L (largest position < L) are
+// then exactly the first sum_{j <= d} C(L, j) entries, so a variant's rank
+// indexes the table directly whatever L is. An entry holds three positions,
+// largest first, one byte each, NO_DEL where the set has fewer; applying all
+// three in that order deletes the set.
+constexpr int DEL_SETS1 = 1 + 18;
+constexpr int DEL_SETS2 = DEL_SETS1 + 153;
+constexpr int DEL_SETS3 = DEL_SETS2 + 816;
+
+struct DelTables {
+	uint32_t d1[DEL_SETS1];
+	uint32_t d2[DEL_SETS2];
+	uint32_t d3[DEL_SETS3];
+};
+
+constexpr uint32_t del_entry(int a, int b, int c) {
+	return (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16);
+}
+
+constexpr DelTables make_del_tables() {
+	DelTables t{};
+	t.d1[0] = t.d2[0] = t.d3[0] = del_entry(NO_DEL, NO_DEL, NO_DEL);
+	int q1 = 1, q2 = 1, q3 = 1;
+	for (int m = 0; m < MAX_PREFIX; ++m) {
+		t.d1[q1++] = del_entry(m, NO_DEL, NO_DEL);
+		t.d2[q2++] = del_entry(m, NO_DEL, NO_DEL);
+		t.d3[q3++] = del_entry(m, NO_DEL, NO_DEL);
+		for (int j = 0; j < m; ++j) {
+			t.d2[q2++] = del_entry(m, j, NO_DEL);
+			t.d3[q3++] = del_entry(m, j, NO_DEL);
+			for (int i = 0; i < j; ++i)
+				t.d3[q3++] = del_entry(m, j, i);
+		}
+	}
+	return t;
+}
+
+__constant__ DelTables c_del_tables = make_del_tables();
 
 // LDS copy of the tables; every thread of the workgroup calls it
 __device__ __forceinline__ void load_gen_tables(GenTables* s, int t, int nt) {
@@ -236,7 +284,6 @@ __device__ __forceinline__ u128 unrank_variant(u128 p, int L, int m, uint32_t k,
 constexpr int GEN_NT = 256;
 constexpr int GEN_IT = 16;  // GEN_TILE == RS_TILE: a generator tile is a radix tile
 constexpr int GEN_TILE = GEN_NT * GEN_IT;
-constexpr int GEN_LDS_SEQS = 1023;  // sequences of a tile whose offsets are staged in LDS
 
 enum GenMode { GEN_DIRECT = 0, GEN_HIST = 1, GEN_APPEND = 2 };
 
@@ -279,9 +326,15 @@ __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n,
 // writes a strided run per lane).
 template <int MODE>
 __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
-	__shared__ uint64_t s_voff[GEN_LDS_SEQS + 1];
-	__shared__ GenTables s_tab;
-	__shared__ u128 s_hm[MAX_PREFIX], s_lm[MAX_PREFIX];
+	// variant offsets of the tile's sequences minus r0, clamped to
+	// [0, GEN_TILE]; the first one (<= 0) is kept whole in s_v0
+	__shared__ uint16_t s_vrel[GEN_TILE + 1];
+	__shared__ int32_t s_v0;
+	__shared__ union {
+		GenTables tab;              // d >= 4
+		uint32_t del[DEL_SETS3];    // d <= 3
+	} s_u;
+	__shared__ u128 s_hm[MAX_PREFIX + 1], s_lm[MAX_PREFIX + 1];
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
 	__shared__ uint32_t s_hist[MODE == GEN_HIST ? 4096 : 1];
@@ -293,7 +346,15 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		return;
 	const int64_t r1 = r0 + GEN_TILE < a.V ? r0 + GEN_TILE : a.V;
 
-	load_gen_tables(&s_tab, t, GEN_NT);
+	const bool small_d = a.distance <= 3;
+	if (small_d) {
+		const uint32_t* src = a.distance == 1 ? c_del_tables.d1 : a.distance == 2 ? c_del_tables.d2 : c_del_tables.d3;
+		const int ns = a.distance == 1 ? DEL_SETS1 : a.distance == 2 ? DEL_SETS2 : DEL_SETS3;
+		for (int i = t; i < ns; i += GEN_NT)
+			s_u.del[i] = src[i];
+	} else {
+		load_gen_tables(&s_u.tab, t, GEN_NT);
+	}
 	init_slot_masks(s_hm, s_lm, t, GEN_NT);
 	if (MODE == GEN_DIRECT)
 		for (int i = t; i < RS_BINS; i += GEN_NT)
@@ -318,14 +379,15 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	__syncthreads();
 	const int64_t slo = s_lo;
 	const int cnt = s_cnt;  // sequences touching this tile, <= GEN_TILE
-	// tiles of more than GEN_LDS_SEQS (very short) sequences search voff in
-	// global memory instead of the LDS copy
-	const bool lds = cnt <= GEN_LDS_SEQS;
-	const uint64_t* vo = lds ? s_voff : a.voff + slo;
-	if (lds)
-		for (int i = t; i <= cnt; i += GEN_NT)
-			s_voff[i] = a.voff[slo + i];
+	// a sequence has at most 2^18 variants: the first offset fits 32 bits
+	for (int i = t; i <= cnt; i += GEN_NT) {
+		const int64_t rel = (int64_t)a.voff[slo + i] - r0;
+		s_vrel[i] = (uint16_t)(rel < 0 ? 0 : (rel > GEN_TILE ? GEN_TILE : rel));
+		if (i == 0)
+			s_v0 = (int32_t)rel;
+	}
 	__syncthreads();
+	const int v0 = s_v0;
 
 	const uint64_t idmask = (1ull << a.idb) - 1ull;
 	int idx = 0;
@@ -335,33 +397,44 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	// would not fit the instruction cache
 #pragma unroll 1
 	for (int it = 0; it < GEN_IT; ++it) {
-		const int64_t r = r0 + (int64_t)it * GEN_NT + t;
+		const int rr = it * GEN_NT + t;  // record index within the tile
+		const int64_t r = r0 + rr;
 		const bool valid = r < r1;
 		uint64_t rec = 0;
 		if (valid) {
 			// records of a lane only move forward: advance the sequence cursor
 			if (it == 0) {
-				int lo = 0, hi = cnt;  // largest idx with vo[idx] <= r
+				int lo = 0, hi = cnt;  // largest idx with s_vrel[idx] <= rr
 				while (lo < hi) {
 					int mid = (lo + hi + 1) >> 1;
-					if (vo[mid] <= (uint64_t)r)
+					if (s_vrel[mid] <= rr)
 						lo = mid;
 					else
 						hi = mid - 1;
 				}
 				idx = lo;
 			} else {
-				while (idx + 1 < cnt && vo[idx + 1] <= (uint64_t)r)
+				while (idx + 1 < cnt && s_vrel[idx + 1] <= rr)
 					++idx;
 			}
 			const int64_t sid = slo + idx;
-			const uint32_t k = (uint32_t)((uint64_t)r - vo[idx]);
+			const uint32_t k = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_vrel[idx]));
 			const uint64_t lo64 = a.pk[2 * sid];
 			const uint64_t hiw = a.pk[2 * sid + 1];
-			const int L = (int)(hiw >> 56);
-			const int m = a.distance < L ? a.distance : L;
 			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
-			const u128 v = unrank_variant(p, L, m, k, s_tab.binom, s_hm, s_lm, s_tab.pair, s_tab.triple);
+			u128 v;
+			if (small_d) {  // uniform: branch-free unranking
+				const uint32_t e = s_u.del[k];
+				v = del_slot_t(p, (int)(e & 0xFFu), s_hm, s_lm);
+				if (a.distance >= 2)
+					v = del_slot_t(v, (int)((e >> 8) & 0xFFu), s_hm, s_lm);
+				if (a.distance >= 3)
+					v = del_slot_t(v, (int)(e >> 16), s_hm, s_lm);
+			} else {
+				const int L = (int)(hiw >> 56);
+				const int m = a.distance < L ? a.distance : L;
+				v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
+			}
 			uint64_t hv = hash_variant(v);
 			if (a.set_of)  // variants of different query sets get different keys
 				hv ^= mix64(((uint64_t)a.set_of[sid] + 1ull) * 0x9E3779B97F4A7C15ull);
