@@ -794,10 +794,25 @@ constexpr int BK_STAGE = 384;  // pairs staged in LDS between flushes
 // the global buffer with one atomic per flush. Every loop is bounded by the
 // bucket count; nothing waits on another workgroup. 52 KB of LDS: three
 // workgroups (24 waves) per CU.
+// records of bucket b into registers (nothing for an overflowing bucket)
+__device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t (&k)[BK_PER],
+                                            uint32_t* s, uint32_t* e) {
+	*s = 0;
+	*e = 0;
+	if (b < a.nb)
+		bucket_range(a, b, s, e);
+	const int c = (int)(*e - *s);
+#pragma unroll
+	for (int i = 0; i < BK_PER; ++i) {
+		const int j = i * BK_NT + (int)threadIdx.x;
+		k[i] = (c <= a.cap && j < c) ? a.rec[*s + j] : 0ull;
+	}
+}
+
 __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	__shared__ uint64_t s_keys[RS_TILE];
 	__shared__ uint16_t s_next[RS_TILE];
-	__shared__ uint32_t s_head[BK_BINS];
+	__shared__ uint32_t s_head[BK_BINS];  // (iteration << 16) | record of the bin's last push
 	__shared__ uint64_t s_stage[BK_STAGE];
 	__shared__ uint32_t s_nstage;
 	__shared__ unsigned long long s_base;
@@ -805,36 +820,30 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	const uint32_t bmask = (1u << a.bin_bits) - 1u;
 	if (t == 0)
 		s_nstage = 0;
-	uint64_t kn[BK_PER];
-	uint32_t s0 = 0, e0 = 0;
-	int b = blockIdx.x;
-	if (b < a.nb) {
-		bucket_range(a, b, &s0, &e0);
-		const int c = (int)(e0 - s0);
-#pragma unroll
-		for (int i = 0; i < BK_PER; ++i) {
-			const int j = i * BK_NT + t;
-			kn[i] = (c <= a.cap && j < c) ? a.rec[s0 + j] : 0ull;
-		}
-	}
+	// heads carry the iteration that pushed them, so a head left by an
+	// earlier bucket reads as an empty bin and the bins need no clearing;
+	// a workgroup handles far fewer than 0xFFFF buckets
+	for (int i = t; i < BK_BINS; i += BK_NT)
+		s_head[i] = 0xFFFF0000u;
+	// two buckets in flight ahead of the one being binned
+	uint64_t k1[BK_PER], k2[BK_PER];
+	uint32_t s1, e1, s2, e2;
+	const int G = (int)gridDim.x;
+	bucket_load(a, blockIdx.x, k1, &s1, &e1);
+	bucket_load(a, blockIdx.x + G, k2, &s2, &e2);
 	__syncthreads();
-	while (b < a.nb) {  // uniform over the workgroup
+	uint32_t iter = 0;
+	for (int b = blockIdx.x; b < a.nb; b += G, ++iter) {  // uniform over the workgroup
 		uint64_t k[BK_PER];
 #pragma unroll
-		for (int i = 0; i < BK_PER; ++i)
-			k[i] = kn[i];
-		const int cnt = (int)(e0 - s0);
-		const int bn = b + (int)gridDim.x;
-		uint32_t s1 = 0, e1 = 0;
-		if (bn < a.nb) {  // next bucket's records in flight during this one
-			bucket_range(a, bn, &s1, &e1);
-			const int c = (int)(e1 - s1);
-#pragma unroll
-			for (int i = 0; i < BK_PER; ++i) {
-				const int j = i * BK_NT + t;
-				kn[i] = (c <= a.cap && j < c) ? a.rec[s1 + j] : 0ull;
-			}
+		for (int i = 0; i < BK_PER; ++i) {
+			k[i] = k1[i];
+			k1[i] = k2[i];
 		}
+		const int cnt = (int)(e1 - s1);
+		s1 = s2;
+		e1 = e2;
+		bucket_load(a, b + 2 * G, k2, &s2, &e2);
 		if (cnt > a.cap) {
 			if (t == 0) {
 				const unsigned long long slot = atomicAdd(a.n_ovf, 1ull);
@@ -842,20 +851,15 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 					a.ovf[slot] = (uint32_t)b;
 			}
 		} else if (cnt >= 2) {
-			for (int i = t; i < BK_BINS; i += BK_NT)
-				s_head[i] = 0xFFFFFFFFu;
+			const uint32_t tag = iter << 16;
 #pragma unroll
 			for (int i = 0; i < BK_PER; ++i) {
 				const int j = i * BK_NT + t;
-				if (j < cnt)
+				if (j < cnt) {
 					s_keys[j] = k[i];
-			}
-			__syncthreads();
-#pragma unroll
-			for (int i = 0; i < BK_PER; ++i) {
-				const int j = i * BK_NT + t;
-				if (j < cnt)
-					s_next[j] = (uint16_t)atomicExch(&s_head[(uint32_t)(k[i] >> a.bin_shift) & bmask], (uint32_t)j);
+					const uint32_t prev = atomicExch(&s_head[(uint32_t)(k[i] >> a.bin_shift) & bmask], tag | (uint32_t)j);
+					s_next[j] = (prev & 0xFFFF0000u) == tag ? (uint16_t)prev : BK_NONE;
+				}
 			}
 			__syncthreads();
 #pragma unroll
@@ -880,7 +884,7 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 			__syncthreads();
 		}
 		const uint32_t ns = s_nstage;
-		if (ns > (uint32_t)BK_STAGE / 2 || (bn >= a.nb && ns > 0)) {  // uniform
+		if (ns > (uint32_t)BK_STAGE / 2 || (b + G >= a.nb && ns > 0)) {  // uniform
 			const uint32_t nf = ns < (uint32_t)BK_STAGE ? ns : (uint32_t)BK_STAGE;
 			__syncthreads();
 			if (t == 0)
@@ -895,9 +899,6 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 				s_nstage = 0;
 			__syncthreads();
 		}
-		b = bn;
-		s0 = s1;
-		e0 = e1;
 	}
 }
 
