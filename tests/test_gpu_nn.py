@@ -93,6 +93,24 @@ def test_bucket_overflow_fallback(engine, n, cap):
     assert np.array_equal(base, oracle.nn_port(chars, offsets, 2, 0)[0])
 
 
+@pytest.mark.parametrize("one_row_heavy", [False, True])
+def test_many_duplicate_candidates(engine, one_row_heavy):
+    # more than 2^19 candidates, most of them repeats of one pair: 20
+    # sequences 60 times each, shuffled; or 400 copies of one sequence at the
+    # front, so that row 0 alone holds ~6000 candidates
+    rng = np.random.default_rng(int(one_row_heavy))
+    extra = clustered_seqs(rng, 3000, 10, 20, AA20, edits=1)
+    if one_row_heavy:
+        seqs = ["CASSLGQGAYEQYF"] * 400 + extra
+    else:
+        seqs = [s for s in random_seqs(rng, 20, 12, 18) for _ in range(60)] + extra
+        seqs = [seqs[i] for i in rng.permutation(len(seqs))]
+    chars, offsets = xtn.encode(seqs)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=1)
+    assert engine.stats()["n_candidates"] > (1 << 19)
+    assert np.array_equal(got, oracle.nn_brute(chars, offsets, 1, 0))
+
+
 @pytest.mark.parametrize("chunk", [1 << 16, 70000])
 def test_multi_chunk_equals_single(engine, chunk):
     chars, offsets = xtn.synth_nn(3, 20000, planted=0.25, max_edits=2)

@@ -40,7 +40,7 @@ struct RunCtrs {
 	unsigned long long n_cand;       // candidate pairs written (append cursor)
 	unsigned long long n_unique;     // unique candidate pairs verified
 	unsigned long long append;       // partition append cursor
-	unsigned long long seg_ovf;      // first-pass segments that outgrew their capacity
+	unsigned long long pad;
 };
 
 struct Run {
@@ -648,83 +648,32 @@ __global__ __launch_bounds__(256) void k_bucket_bounds(const uint64_t* __restric
 // bucket as overflowing (more than the LDS bucket kernel holds), and the
 // chunk is then redone on the exact two-pass sort.
 
-// First-pass segments: the records of coarse bucket c (the top 8 bucket bits)
-// are either one contiguous range (exact pass) or XCD_PARTS sub-regions of
-// fixed capacity (the generator's coarse scatter), segment s then covering
-// coarse bucket s / XCD_PARTS. k_seg_tiles lays out, in one u32 buffer:
-//   [0, nseg)          first record of each segment
-//   [nseg, 2 nseg)     records of each segment
-//   [2 nseg]           tiles in all
-//   [2 nseg + 1, ...)  per tile: segment | tile index inside it << 11
-constexpr int XCD_PARTS = 8;
-constexpr int NSUB = RS_BINS * XCD_PARTS;
-constexpr int SEG_NT = 1024;
+// ctl[0..256] = first record of each coarse bucket (cstart, cstart[256] = n),
+// ctl[257..513] = first tile of each coarse bucket (tstart[256] = tiles)
+constexpr int CTL_WORDS = 2 * (RS_BINS + 1);
 
-__host__ __device__ constexpr int64_t seg_ctl_words(int64_t nrec, int nseg) {
-	return 2 * (int64_t)nseg + 1 + (nrec + RS_TILE - 1) / RS_TILE + nseg;
+__global__ __launch_bounds__(RS_BINS) void k_coarse_tiles(const uint32_t* __restrict__ dtot,
+                                                          uint32_t* __restrict__ ctl) {
+	__shared__ uint32_t s_tmp[RS_BINS / WAVE + 1];
+	const int c = threadIdx.x;
+	const uint32_t cnt = dtot[c];
+	uint32_t n;
+	const uint32_t cs = block_excl_scan<RS_BINS>(cnt, s_tmp, &n);
+	const uint32_t tiles = (cnt + RS_TILE - 1) / RS_TILE;
+	uint32_t nt;
+	const uint32_t ts = block_excl_scan<RS_BINS>(tiles, s_tmp, &nt);
+	ctl[c] = cs;
+	ctl[RS_BINS + 1 + c] = ts;
+	if (c == 0) {
+		ctl[RS_BINS] = n;
+		ctl[2 * RS_BINS + 1] = nt;
+	}
 }
 
-// counts[s] records per segment; subcap > 0: segment s starts at s * subcap
-// and holds at most subcap records (more count as an overflow), else the
-// segments are contiguous in order
-__global__ __launch_bounds__(SEG_NT) void k_seg_tiles(const uint32_t* __restrict__ counts, int nseg,
-                                                      int64_t subcap, uint32_t* __restrict__ ctl,
-                                                      unsigned long long* __restrict__ ovf) {
-	__shared__ uint32_t s_tmp[SEG_NT / WAVE + 1];
-	const int t = threadIdx.x;
-	const int per = (nseg + SEG_NT - 1) / SEG_NT;  // <= 2
-	uint32_t c[2] = {0, 0}, nt[2] = {0, 0};
-	uint32_t csum = 0, tsum = 0, over = 0;
-	for (int q = 0; q < per; ++q) {
-		const int sg = t * per + q;
-		if (sg >= nseg)
-			break;
-		uint32_t v = counts[sg];
-		if (subcap > 0 && (int64_t)v > subcap) {
-			++over;
-			v = (uint32_t)subcap;
-		}
-		c[q] = v;
-		nt[q] = (v + RS_TILE - 1) / RS_TILE;
-		csum += v;
-		tsum += nt[q];
-	}
-	uint32_t ntot;
-	uint32_t cbase = block_excl_scan<SEG_NT>(csum, s_tmp, (uint32_t*)nullptr);
-	uint32_t tbase = block_excl_scan<SEG_NT>(tsum, s_tmp, &ntot);
-	if (over)
-		atomicAdd(ovf, (unsigned long long)over);
-	uint32_t* tiles = ctl + 2 * nseg + 1;
-	for (int q = 0; q < per; ++q) {
-		const int sg = t * per + q;
-		if (sg >= nseg)
-			break;
-		ctl[sg] = subcap > 0 ? (uint32_t)((int64_t)sg * subcap) : cbase;
-		ctl[nseg + sg] = c[q];
-		for (uint32_t k = 0; k < nt[q]; ++k)
-			tiles[tbase + k] = (uint32_t)sg | (k << 11);
-		cbase += c[q];
-		tbase += nt[q];
-	}
-	if (t == 0)
-		ctl[2 * nseg] = ntot;
-}
-
-// Buckets of the B > 8 bucket bits are laid out as fixed regions of
-// `stride` slots: bucket b owns [b * stride, b * stride + count[b]). The
-// records come grouped by coarse bucket (first-pass segments), tiles are cut
-// inside segments, so every record of a tile shares c; a tile ranks its
-// records by the remaining bucket bits in LDS and claims each run with one
-// atomic on the bucket's cursor (16 records per atomic on average). Slots past
-// the stride are not written: the cursor still counts them, which marks the
-// bucket as overflowing (more than the LDS bucket kernel holds), and the
-// chunk is then redone on the exact two-pass sort.
 struct RegionArgs {
-	const uint64_t* in;     // records grouped by first-pass segment
+	const uint64_t* in;     // sorted by the coarse digit
 	uint64_t* out;          // the regions
-	const uint32_t* ctl;    // k_seg_tiles
-	int32_t nseg;
-	int32_t seg_shift;      // coarse bucket = segment >> seg_shift
+	const uint32_t* ctl;    // k_coarse_tiles
 	uint32_t* count;        // 2^B cursors, zeroed
 	int64_t stride;         // slots per region
 	int32_t shift;          // lowest bucket bit: the bucket is bits [shift, shift + B)
@@ -737,21 +686,30 @@ __global__ __launch_bounds__(RS_NT) void k_region_scatter(RegionArgs a) {
 	__shared__ uint32_t s_dpre[RS_BINS + 1];
 	__shared__ uint32_t s_gbase[RS_BINS];
 	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+	__shared__ uint32_t s_ts[RS_BINS + 1];
 
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
-	const uint32_t ntiles = a.ctl[2 * a.nseg];
+	for (int i = t; i <= RS_BINS; i += RS_NT)
+		s_ts[i] = a.ctl[RS_BINS + 1 + i];
+	__syncthreads();
+	const uint32_t ntiles = s_ts[RS_BINS];
 	const uint32_t tile = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
 	if (tile >= ntiles)  // uniform
 		return;
-	const uint32_t te = a.ctl[2 * a.nseg + 1 + tile];
-	const uint32_t sg = te & 0x7FFu;
-	const uint32_t c = sg >> a.seg_shift;
-	const uint32_t kt = te >> 11;
-	const int64_t base = (int64_t)a.ctl[sg] + (int64_t)kt * RS_TILE;
-	const int64_t left = (int64_t)a.ctl[a.nseg + sg] - (int64_t)kt * RS_TILE;
-	const int tile_n = (int)(left < RS_TILE ? left : RS_TILE);  // >= 1
+	int lo = 0, hi = RS_BINS - 1;  // the coarse bucket: largest c with s_ts[c] <= tile
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (s_ts[mid] <= tile)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	const uint32_t c = (uint32_t)lo;
+	const int64_t cbeg = a.ctl[c], cend = a.ctl[c + 1];
+	const int64_t base = cbeg + (int64_t)(tile - s_ts[c]) * RS_TILE;
+	const int tile_n = (int)(cend - base < RS_TILE ? cend - base : RS_TILE);  // >= 1
 
 	const int64_t seg = base + (int64_t)w * RS_SEG;
 	const int wrem = tile_n - w * RS_SEG;
@@ -784,150 +742,6 @@ __global__ __launch_bounds__(RS_NT) void k_region_scatter(RegionArgs a) {
 		const uint32_t slot = s_gbase[d] + ((uint32_t)i - s_dpre[d]);
 		if ((int64_t)slot < a.stride)
 			a.out[(int64_t)((c << a.fine_bits) | (d & fmask)) * a.stride + slot] = key;
-	}
-}
-
-// Generator fused with the exact first pass (one key-space chunk, d <= 3):
-// a workgroup generates its GEN_TILE records into registers (the loop
-// unrolled; d <= 3 unranks branch-free from the deletion-set tables), ranks
-// them by the coarse digit (bits [shift, shift + 8)) in LDS, and appends each
-// digit run to sub-region (digit, blockIdx % XCD_PARTS) with one atomic. Tiles
-// of one XCD then feed the same 256 sub-regions, whose partial lines merge in
-// that XCD's L2. Records past a sub-region's capacity are not written; the
-// cursor still counts them (k_seg_tiles reports the overflow).
-struct CoarseArgs {
-	uint32_t* sub_cnt;  // NSUB cursors, zeroed
-	int64_t subcap;     // slots per sub-region
-	int32_t shift;
-	int32_t pad;
-};
-
-__global__ __launch_bounds__(GEN_NT) void k_gen_coarse(GenArgs a, CoarseArgs ca) {
-	__shared__ union {
-		struct {
-			uint16_t vrel[GEN_TILE + 1];
-			uint32_t del[DEL_SETS3];
-		} g;                        // while generating
-		uint64_t keys[RS_TILE];     // while scattering
-	} s_u;
-	__shared__ u128 s_hm[MAX_PREFIX + 1], s_lm[MAX_PREFIX + 1];
-	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
-	__shared__ uint32_t s_dpre[RS_BINS + 1];
-	__shared__ uint32_t s_gbase[RS_BINS];
-	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
-	__shared__ int64_t s_lo;
-	__shared__ int s_nseq;
-	__shared__ int32_t s_v0;
-	static_assert(GEN_NT == RS_NT && GEN_IT == RS_IT, "a generator tile is a radix tile");
-
-	const int t = threadIdx.x;
-	const int64_t r0 = (int64_t)blockIdx.x * GEN_TILE;
-	if (r0 >= a.V)
-		return;
-	const int64_t r1 = r0 + GEN_TILE < a.V ? r0 + GEN_TILE : a.V;
-	const int tile_n = (int)(r1 - r0);
-	{
-		const uint32_t* src = a.distance == 1 ? c_del_tables.d1 : a.distance == 2 ? c_del_tables.d2 : c_del_tables.d3;
-		const int ns = a.distance == 1 ? DEL_SETS1 : a.distance == 2 ? DEL_SETS2 : DEL_SETS3;
-		for (int i = t; i < ns; i += GEN_NT)
-			s_u.g.del[i] = src[i];
-	}
-	init_slot_masks(s_hm, s_lm, t, GEN_NT);
-	if (t == 0) {  // as in k_gen
-		int64_t lo = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)r0) - 1;
-		int64_t hi = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)(r1 - 1)) - 1;
-		lo = lo < a.n - 1 ? lo : a.n - 1;
-		hi = hi < a.n - 1 ? hi : a.n - 1;
-		hi = hi > lo ? hi : lo;
-		hi = hi - lo + 1 <= GEN_TILE ? hi : lo + GEN_TILE - 1;
-		s_lo = lo;
-		s_nseq = (int)(hi - lo + 1);
-	}
-	__syncthreads();
-	const int64_t slo = s_lo;
-	const int cnt = s_nseq;
-	for (int i = t; i <= cnt; i += GEN_NT) {
-		const int64_t rel = (int64_t)a.voff[slo + i] - r0;
-		s_u.g.vrel[i] = (uint16_t)(rel < 0 ? 0 : (rel > GEN_TILE ? GEN_TILE : rel));
-		if (i == 0)
-			s_v0 = (int32_t)rel;
-	}
-	__syncthreads();
-	const int v0 = s_v0;
-
-	const uint64_t idmask = (1ull << a.idb) - 1ull;
-	uint64_t k[GEN_IT];
-	int idx = 0;
-#pragma unroll
-	for (int it = 0; it < GEN_IT; ++it) {
-		const int rr = it * GEN_NT + t;
-		uint64_t rec = 0;
-		if (rr < tile_n) {
-			if (it == 0) {
-				int lo = 0, hi = cnt;
-				while (lo < hi) {
-					int mid = (lo + hi + 1) >> 1;
-					if (s_u.g.vrel[mid] <= rr)
-						lo = mid;
-					else
-						hi = mid - 1;
-				}
-				idx = lo;
-			} else {
-				while (idx + 1 < cnt && s_u.g.vrel[idx + 1] <= rr)
-					++idx;
-			}
-			const int64_t sid = slo + idx;
-			const uint32_t kk = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_u.g.vrel[idx]));
-			const uint64_t lo64 = a.pk[2 * sid];
-			const uint64_t hiw = a.pk[2 * sid + 1];
-			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
-			const uint32_t e = s_u.g.del[kk];
-			u128 v = del_slot_t(p, (int)(e & 0xFFu), s_hm, s_lm);
-			if (a.distance >= 2)
-				v = del_slot_t(v, (int)((e >> 8) & 0xFFu), s_hm, s_lm);
-			if (a.distance >= 3)
-				v = del_slot_t(v, (int)(e >> 16), s_hm, s_lm);
-			uint64_t hv = hash_variant(v);
-			if (a.set_of)
-				hv ^= mix64(((uint64_t)a.set_of[sid] + 1ull) * 0x9E3779B97F4A7C15ull);
-			rec = (hv & ~idmask) | ((uint64_t)sid & idmask);
-		}
-		k[it] = rec;
-	}
-	// Thread (w, lane) holds record it * GEN_NT + w * WAVE + lane in k[it];
-	// read as rank_tile's position w * RS_SEG + it * WAVE + lane (the order
-	// inside a sub-region is free), it is valid exactly when
-	// it * WAVE + lane < w_valid:
-	const int w = t / WAVE;
-	const int m = tile_n - w * WAVE;
-	int w_valid = 0;
-	if (m > 0) {
-		const int q = m / GEN_NT, rem = m - q * GEN_NT;
-		w_valid = q * WAVE + (rem < WAVE ? rem : WAVE);
-		w_valid = w_valid < RS_SEG ? w_valid : RS_SEG;
-	}
-	uint32_t lp[RS_IT];
-	rank_tile(k, w_valid, ca.shift, lp, s_cnt, s_dpre, s_scan);  // generation is over past its barriers
-#pragma unroll
-	for (int r = 0; r < RS_IT; ++r)
-		if (lp[r] != 0xFFFFFFFFu)
-			s_u.keys[lp[r]] = k[r];
-	if (t == 0)
-		s_dpre[RS_BINS] = (uint32_t)tile_n;
-	__syncthreads();
-	const uint32_t part = blockIdx.x % XCD_PARTS;
-	{
-		const uint32_t c = s_dpre[t + 1] - s_dpre[t];
-		s_gbase[t] = c ? atomicAdd(&ca.sub_cnt[t * XCD_PARTS + part], c) : 0u;
-	}
-	__syncthreads();
-	for (int i = t; i < tile_n; i += GEN_NT) {
-		const uint64_t key = s_u.keys[i];
-		const uint32_t d = (uint32_t)(key >> ca.shift) & 0xFFu;
-		const uint32_t slot = s_gbase[d] + ((uint32_t)i - s_dpre[d]);
-		if ((int64_t)slot < ca.subcap)
-			a.out[(int64_t)(d * XCD_PARTS + part) * ca.subcap + slot] = key;
 	}
 }
 

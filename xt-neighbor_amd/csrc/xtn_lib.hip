@@ -73,7 +73,7 @@ struct xtn_handle {
 	int64_t n_chars = 0;
 
 	// workspaces (grow-only)
-	DBuf bcount, ctl, subcnt;  // region layout: bucket cursors, segment/tile table, sub-region cursors
+	DBuf bcount, ctl;  // region layout: bucket cursors, coarse bucket/tile table
 	DBuf pk, voff, recs, recs_alt, hist, scan_tmp, runs_small, runs_big, big_pre, pairs, pairs_alt,
 	    flags, pos, dist, out_pairs, out_dist, ctrs, part_hist, scalar;
 	// overlap-mode workspaces
@@ -614,7 +614,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	TRY(ensure(h, h->runs_big, sizeof(Run) * (size_t)(cap_recs / (SMALL_RUN_MAX + 1) + 1)));
 	TRY(ensure(h, h->bstart, sizeof(uint32_t) * (65536 + 1)));
 	TRY(ensure(h, h->bcount, sizeof(uint32_t) * 65536));
-	TRY(ensure(h, h->subcnt, sizeof(uint32_t) * NSUB));
+	TRY(ensure(h, h->ctl, sizeof(uint32_t) * CTL_WORDS));
 	TRY(ensure(h, h->ovf, sizeof(uint32_t) * 65536));
 	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, RS_TILE) : RS_TILE;
 
@@ -638,35 +638,28 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			++B;
 		B = std::min(B, K);
 		const int SB = (B + 7) / 8 * 8;
-		// more than 8 bucket bits: a first pass groups the records by the top 8
-		// (coarse buckets), then the region scatter places them in fixed
-		// regions of bucket_cap slots (in recs). With one chunk and d <= 3 the
-		// generator does the first pass itself (sub-regions in recs_alt).
+		// more than 8 bucket bits: an exact pass over the top 8, then the
+		// region scatter (fixed regions of bucket_cap slots in recs)
 		const bool region = B > 8;
-		const bool coarse = region && nparts == 1 && d <= 3;
-		const int64_t rtiles = (nrec + RS_TILE - 1) / RS_TILE;
-		const int nseg = coarse ? NSUB : RS_BINS;
-		int64_t subcap = 0;
 		if (region) {
 			TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)((int64_t)bucket_cap << B)));
-			TRY(ensure(h, h->ctl, sizeof(uint32_t) * (size_t)seg_ctl_words(nrec, nseg)));
-		}
-		if (coarse) {  // sub-regions: the mean plus 25 % plus one tile
-			subcap = nrec / NSUB + nrec / (4 * NSUB) + RS_TILE;
-			TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)(NSUB * subcap)));
-		}
-		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
-		TRY(set_ctr(h, offsetof(RunCtrs, seg_ovf), 0ull));
-		// records in generation order into recs; with fuse, the generator also
-		// counts the digit at digit_shift per tile
-		auto generate_linear = [&](bool fuse, int digit_shift) -> int {
-			StageScope sg(h, "generate");
 			ga.out = h->recs.as<uint64_t>();
-			ga.digit_hist = fuse ? h->hist.as<uint32_t>() : nullptr;
+		}
+		// the single-chunk generator also counts the first sort digit per tile
+		const bool fused_hist = nparts == 1 && B > 0;
+		const int64_t rtiles = (nrec + RS_TILE - 1) / RS_TILE;
+		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
+		if (fused_hist) {
+			ga.digit_hist = h->hist.as<uint32_t>();
 			ga.ntiles = rtiles;
-			ga.digit_shift = digit_shift;
+			ga.digit_shift = region ? hi - 8 : hi - SB;
+		} else {
+			ga.digit_hist = nullptr;
+		}
+		{
+			StageScope sg(h, "generate");
 			if (nparts == 1) {
-				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (fuse ? 1024.0 * rtiles : 0.0));
+				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (fused_hist ? 1024.0 * rtiles : 0.0));
 				k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
 				HIP_LAUNCH_CHECK(h);
 			} else {
@@ -677,29 +670,6 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				k_gen<GEN_APPEND><<<gen_grid, GEN_NT, 0, s>>>(ga);
 				HIP_LAUNCH_CHECK(h);
 			}
-			return XTN_OK;
-		};
-		// the single-chunk generator counts the first sort digit per tile
-		const bool fused_hist = nparts == 1 && B > 0;
-		if (coarse) {
-			StageScope sg(h, "generate");
-			k_fill_u64<<<grid_for(NSUB / 2, 256), 256, 0, s>>>(h->subcnt.as<unsigned long long>(), NSUB / 2, 0ull);
-			HIP_LAUNCH_CHECK(h);
-			ga.out = h->recs_alt.as<uint64_t>();
-			CoarseArgs ca{};
-			ca.sub_cnt = h->subcnt.as<uint32_t>();
-			ca.subcap = subcap;
-			ca.shift = hi - 8;
-			{
-				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0);
-				k_gen_coarse<<<gen_grid, GEN_NT, 0, s>>>(ga, ca);
-				HIP_LAUNCH_CHECK(h);
-			}
-			k_seg_tiles<<<1, SEG_NT, 0, s>>>(h->subcnt.as<uint32_t>(), NSUB, subcap, h->ctl.as<uint32_t>(),
-			                                 ctr_field(h, offsetof(RunCtrs, seg_ovf)));
-			HIP_LAUNCH_CHECK(h);
-		} else {
-			TRY(generate_linear(fused_hist, region ? hi - 8 : hi - SB));
 		}
 		const int nb = 1 << B;
 		BucketArgs ba{};
@@ -752,40 +722,38 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 
 		bool sorted_layout = true;
 		if (region) {
-			if (!coarse) {  // exact first pass: recs -> recs_alt, stable by the top 8 bucket bits
-				StageScope sg(h, "sort");
-				uint32_t* hist = h->hist.as<uint32_t>();
-				if (!fused_hist) {
-					ProfScope ps(h, "sort_rec_hist", (double)nrec * 8.0);
-					k_radix_hist<<<(unsigned)rtiles, RS_NT, 0, s>>>(h->recs.as<uint64_t>(), nrec, hi - 8, hist, rtiles);
-					HIP_LAUNCH_CHECK(h);
-				}
-				uint32_t* dtot = nullptr;
-				TRY(digit_scan(h, hist, rtiles, &dtot));
-				k_seg_tiles<<<1, SEG_NT, 0, s>>>(dtot, RS_BINS, 0, h->ctl.as<uint32_t>(),
-				                                 ctr_field(h, offsetof(RunCtrs, seg_ovf)));
+			// exact pass over the top 8 bucket bits, then the region scatter
+			// over the rest; buckets are read from their regions
+			StageScope sg(h, "sort");
+			uint32_t* hist = h->hist.as<uint32_t>();
+			if (!fused_hist) {
+				ProfScope ps(h, "sort_rec_hist", (double)nrec * 8.0);
+				k_radix_hist<<<(unsigned)rtiles, RS_NT, 0, s>>>(h->recs.as<uint64_t>(), nrec, hi - 8, hist, rtiles);
 				HIP_LAUNCH_CHECK(h);
+			}
+			uint32_t* dtot = nullptr;
+			TRY(digit_scan(h, hist, rtiles, &dtot));
+			k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
+			HIP_LAUNCH_CHECK(h);
+			{
 				ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)nrec);
 				k_radix_scatter<false><<<(unsigned)rtiles, RS_NT, 0, s>>>(
 				    h->recs.as<uint64_t>(), nullptr, h->recs_alt.as<uint64_t>(), nullptr, nrec, hi - 8, hist, rtiles);
 				HIP_LAUNCH_CHECK(h);
 			}
+			k_fill_u64<<<grid_for(nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), nb / 2, 0ull);
+			HIP_LAUNCH_CHECK(h);
+			RegionArgs ra{};
+			ra.in = h->recs_alt.as<uint64_t>();
+			ra.out = h->recs.as<uint64_t>();
+			ra.ctl = h->ctl.as<uint32_t>();
+			ra.count = h->bcount.as<uint32_t>();
+			ra.stride = bucket_cap;
+			ra.shift = hi - B;
+			ra.fine_bits = B - 8;
 			{
-				StageScope sg(h, "sort");
-				k_fill_u64<<<grid_for(nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), nb / 2, 0ull);
-				HIP_LAUNCH_CHECK(h);
-				RegionArgs ra{};
-				ra.in = h->recs_alt.as<uint64_t>();
-				ra.out = h->recs.as<uint64_t>();
-				ra.ctl = h->ctl.as<uint32_t>();
-				ra.nseg = nseg;
-				ra.seg_shift = coarse ? 3 : 0;
-				ra.count = h->bcount.as<uint32_t>();
-				ra.stride = bucket_cap;
-				ra.shift = hi - B;
-				ra.fine_bits = B - 8;
 				ProfScope ps(h, "region", 16.0 * (double)nrec);
-				k_region_scatter<<<(unsigned)(rtiles + nseg), RS_NT, 0, s>>>(ra);
+				k_region_scatter<<<(unsigned)(rtiles + RS_BINS), RS_NT, 0, s>>>(ra);
 				HIP_LAUNCH_CHECK(h);
 			}
 			ba.rec = h->recs.as<uint64_t>();
@@ -793,13 +761,10 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ba.bstride = bucket_cap;
 			ba.bstart = nullptr;
 			TRY(run_buckets());
-			if (rc.append > 0 || rc.seg_ovf > 0) {
-				// a first-pass segment or a bucket outgrew its space: redo the
-				// chunk on the exact two-pass sort
-				if (coarse)
-					TRY(generate_linear(false, 0));
-				else
-					std::swap(h->recs, h->recs_alt);  // the first pass's output
+			if (rc.append > 0) {
+				// a bucket outgrew its region: redo the chunk on the exact
+				// two-pass sort of the coarse-sorted records
+				std::swap(h->recs, h->recs_alt);
 				ba.bcount = nullptr;
 			} else {
 				sorted_layout = false;
@@ -1112,7 +1077,7 @@ void xtn_close(xtn_handle* h) {
 		return;
 	(void)hipSetDevice(h->cfg.device);
 	(void)hipStreamSynchronize(h->stream);
-	DBuf* bufs[] = {&h->bcount, &h->ctl, &h->subcnt, &h->chars, &h->offs, &h->pk, &h->voff, &h->recs, &h->recs_alt, &h->hist,
+	DBuf* bufs[] = {&h->bcount, &h->ctl, &h->chars, &h->offs, &h->pk, &h->voff, &h->recs, &h->recs_alt, &h->hist,
 	                &h->scan_tmp, &h->runs_small, &h->runs_big, &h->big_pre, &h->pairs,
 	                &h->pairs_alt, &h->flags, &h->pos, &h->dist, &h->out_pairs, &h->out_dist,
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,

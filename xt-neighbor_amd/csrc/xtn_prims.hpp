@@ -320,14 +320,16 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t i, int64_t ntiles) {
 	return xcd * q + (xcd < r ? xcd : r) + local;
 }
 
-// Stable rank of a tile of up to RS_TILE keys by the 8-bit digit at `shift`.
-// Wave w holds the keys of tile positions [w*RS_SEG, (w+1)*RS_SEG), round r
-// lane l holding position w*RS_SEG + r*64 + l. On return lp[r] is the key's
-// position in the tile sorted by digit (stable), or 0xFFFFFFFF when invalid,
-// and s_dpre[d] the first position of digit d. Every thread must call it.
-__device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_valid, int shift,
-                                          uint32_t (&lp)[RS_IT], uint32_t (*s_cnt)[RS_BINS],
-                                          uint32_t* s_dpre, uint32_t* s_scan) {
+// Stable rank of a tile of up to RS_NT * IT keys by the 8-bit digit at
+// `shift`. Wave w holds the keys of tile positions [w*IT*64, (w+1)*IT*64),
+// round r lane l holding position w*IT*64 + r*64 + l; only the first w_valid
+// positions of a wave are keys. On return lp[r] is the key's position in the
+// tile sorted by digit (stable), or 0xFFFFFFFF when invalid, and s_dpre[d]
+// the first position of digit d. Every thread must call it.
+template <int IT>
+__device__ __forceinline__ void rank_tile_n(const uint64_t (&k)[IT], int w_valid, int shift,
+                                            uint32_t (&lp)[IT], uint32_t (*s_cnt)[RS_BINS],
+                                            uint32_t* s_dpre, uint32_t* s_scan) {
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
@@ -335,9 +337,12 @@ __device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_vali
 		(&s_cnt[0][0])[i] = 0;
 	__syncthreads();
 	const uint64_t lt = lanemask_lt();
-	uint32_t rk[RS_IT];
+	uint32_t rk[IT];
 #pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
+	for (int r = 0; r < IT; ++r) {
+		rk[r] = 0xFFFFFFFFu;
+		if (r * WAVE >= w_valid)  // uniform over the wave: no key in this round
+			continue;
 		const bool valid = r * WAVE + lane < w_valid;
 		const uint32_t d = (uint32_t)(k[r] >> shift) & 0xFFu;
 		const uint64_t peers = match_digit8(d, valid);
@@ -368,8 +373,14 @@ __device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_vali
 	}
 	__syncthreads();
 #pragma unroll
-	for (int r = 0; r < RS_IT; ++r)
+	for (int r = 0; r < IT; ++r)
 		lp[r] = rk[r] == 0xFFFFFFFFu ? 0xFFFFFFFFu : s_cnt[w][rk[r] & 0xFFu] + (rk[r] >> 8);
+}
+
+__device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_valid, int shift,
+                                          uint32_t (&lp)[RS_IT], uint32_t (*s_cnt)[RS_BINS],
+                                          uint32_t* s_dpre, uint32_t* s_scan) {
+	rank_tile_n<RS_IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 }
 
 template <bool KV>
