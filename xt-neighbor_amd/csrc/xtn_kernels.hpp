@@ -110,10 +110,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 	return x;
 }
 
+// 90-bit packed variant -> 64-bit key. The 26 high bits enter through an odd
+// multiplier (injective), then one xorshift-multiply-xorshift round spreads
+// every input bit into the high bits the records are grouped by. Two
+// quarter-rate multiplies fewer than a full mix64 of the folded value.
 __device__ __forceinline__ uint64_t hash_variant(u128 v) {
 	const uint64_t lo = (uint64_t)v;
-	const uint64_t hi = (uint64_t)(v >> 64);
-	return mix64(lo ^ (hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull);
+	const uint32_t hi = (uint32_t)(v >> 64);  // 26 bits
+	uint64_t x = lo ^ ((uint64_t)hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull;
+	x ^= x >> 32;
+	x *= 0xD6E8FEB86659FD93ull;
+	x ^= x >> 29;
+	return x;
 }
 
 // del_slot with precomputed masks: hm[pos] keeps slots 0..pos-1, lm[pos]
