@@ -92,15 +92,6 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
 
 // ---------------------------------------------------------- generation --
 
-__device__ __forceinline__ u128 del_slot(u128 p, int pos) {
-	// slot pos occupies bits [5*(17-pos), 5*(17-pos)+5)
-	const int low_bits = 5 * (MAX_PREFIX - 1 - pos);
-	const u128 one = 1;
-	const u128 low_mask = (one << low_bits) - 1;
-	const u128 high_mask = ~((one << (low_bits + 5)) - 1);
-	return (p & high_mask) | ((p & low_mask) << 5);
-}
-
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
 	x ^= x >> 32;
 	x *= 0xD6E8FEB86659FD93ull;
@@ -124,8 +115,10 @@ __device__ __forceinline__ uint64_t hash_variant(u128 v) {
 	return x;
 }
 
-// del_slot with precomputed masks: hm[pos] keeps slots 0..pos-1, lm[pos]
-// keeps slots pos+1..17 (moved up by one slot)
+// Deletes slot pos of a packed prefix (slot pos occupies bits
+// [5*(17-pos), 5*(17-pos)+5)): the slots before it stay, the slots after it
+// move up by one, the last slot becomes empty. hm[pos] keeps slots 0..pos-1,
+// lm[pos] slots pos+1..17.
 __device__ __forceinline__ u128 del_slot_t(u128 p, int pos, const u128* __restrict__ hm,
                                            const u128* __restrict__ lm) {
 	return (p & hm[pos]) | ((p & lm[pos]) << 5);

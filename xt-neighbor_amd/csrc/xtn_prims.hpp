@@ -442,28 +442,5 @@ __global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restr
 	}
 }
 
-// One stable LSD pass over the n <= RS_TILE keys held in s_keys, in place.
-// Every thread of the workgroup must call it.
-__device__ __forceinline__ void lds_radix_pass(uint64_t* s_keys, int n, int shift,
-                                               uint32_t (*s_cnt)[RS_BINS], uint32_t* s_dpre,
-                                               uint32_t* s_scan) {
-	const int lane = lane_id();
-	const int w = threadIdx.x / WAVE;
-	const int wrem = n - w * RS_SEG;
-	const int w_valid = wrem <= 0 ? 0 : (wrem < RS_SEG ? wrem : RS_SEG);
-	uint64_t k[RS_IT];
-#pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
-		const int idx = w * RS_SEG + r * WAVE + lane;
-		k[r] = r * WAVE + lane < w_valid ? s_keys[idx] : 0ull;
-	}
-	uint32_t lp[RS_IT];
-	rank_tile(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
-#pragma unroll
-	for (int r = 0; r < RS_IT; ++r)
-		if (lp[r] != 0xFFFFFFFFu)
-			s_keys[lp[r]] = k[r];
-	__syncthreads();
-}
 
 }  // namespace xtn
