@@ -15,8 +15,9 @@ system ROCm's and would replace the one libxtn.so was built against.
 
 Prints one JSON line. Also reports verified pairs/s, p50/p99 step latency,
 the roofline of the dominant kernel (HIP events around every launch on the
-engine's stream) and a CPU baseline: the oracle's single-thread restatement
-of the reference algorithm timed on a bounded prefix of the same workload.
+engine's stream), a plain copy kernel's GB/s, and a CPU baseline: the
+oracle's restatement of the reference algorithm on the host cores (OpenMP)
+over the same workload, plus its single-thread form on a bounded prefix.
 """
 import argparse
 import ctypes
@@ -52,19 +53,47 @@ def hip_sync():
         raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
 
 
-def cpu_baseline(chars, offsets, d, measure, sample):
+def cpu_baseline(chars, offsets, d, measure, sample, threads):
+    """The oracle's restatement of the reference algorithm on the host cores:
+    the OpenMP port on `threads` threads over the whole workload (median of
+    5 runs), and the single-thread port on a bounded prefix."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    n = min(sample, len(offsets) - 1)
-    sub_off = offsets[: n + 1].copy()
+    n = len(offsets) - 1
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        tri, st = oracle.nn_port_mt(chars, offsets, d, measure, threads)
+        times.append(time.perf_counter() - t0)
+    dt = statistics.median(times)
+    ns = min(sample, n)
+    sub_off = offsets[: ns + 1].copy()
     sub_chars = chars[: int(sub_off[-1])].copy()
     t0 = time.perf_counter()
-    _, st = oracle.nn_port(sub_chars, sub_off, d, measure)
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "seqs/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} sequences of the same workload, d={d}, one thread, "
-                      f"{dt:.1f} s, {st['n_output']} output pairs"}
+    _, st1 = oracle.nn_port(sub_chars, sub_off, d, measure)
+    dt1 = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "seqs/s", "cores": threads, "kind": "port",
+            "sample": f"the whole workload ({n} sequences, d={d}), {threads} OpenMP threads, median of 5 "
+                      f"runs {dt:.2f} s, {st['n_output']} output pairs, {st['n_unique']} unique candidates",
+            "verified_pairs_per_s": st["n_unique"] / dt,
+            "one_thread": {"value": ns / dt1, "unit": "seqs/s", "cores": 1,
+                           "sample": f"first {ns} sequences, {dt1:.1f} s, {st1['n_output']} output pairs"},
+            "host": host_cpu()}
+
+
+def host_cpu() -> str:
+    """model name and core count of the host"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs visible"
 
 
 def batched_vs_plain(eng, offsets, n_sets, seed):
@@ -179,6 +208,64 @@ def end_to_end(chars, offsets, d, measure):
     return out
 
 
+def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int, e2e: bool) -> dict:
+    """Repertoire overlap on the batched-repertoire workload: n_reps
+    repertoires of log-uniform size, Zipf(2) frequencies capped at 1e5, 10%
+    of sequences shared or 1-edit-shared across repertoires, Levenshtein d=1.
+    A run is xtn_perform in overlap mode over all repertoires at once (host
+    SeqInfo in, R x R sums out); with e2e, the CLI on the same rows too."""
+    import subprocess
+    import tempfile
+
+    chars, offsets, freq, sizes = xtn.synth_overlap(seed, n_reps, size_min, size_max, shared=0.1)
+    rep = xtn.reps_from_sizes(sizes)
+    n = len(freq)
+    out = {"repertoires": n_reps, "rows": n, "sizes": [int(size_min), int(size_max)], "distance": 1,
+           "measure": "leven"}
+    with xtn.Engine(distance=1) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        res = eng.overlap(freq, rep)
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            res = eng.overlap(freq, rep)
+            times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    out.update({"ms": t * 1e3, "seqs_per_s": n / t, "output_rows": int(len(res))})
+    if e2e:
+        tmp = tempfile.mkdtemp(prefix="xtn_bench_ov_")
+        seqs, info = os.path.join(tmp, "seqs.csv"), os.path.join(tmp, "info.csv")
+        xtn.write_csv(seqs, chars, offsets, freq)
+        with open(info, "w") as f:
+            f.write("size\n" + "".join(f"{int(x)}\n" for x in sizes))
+        cmd = [xtn.CLI_PATH, "-i", seqs, "-n", str(n), "-I", info, "-N", str(n_reps)]
+        wall = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            r = subprocess.run(cmd, capture_output=True)
+            wall.append(time.perf_counter() - t0)
+            if r.returncode != 0:
+                raise RuntimeError(r.stderr.decode())
+        out["cli_no_output_s"] = min(wall)
+        out["cli_seqs_per_s"] = n / min(wall)
+        for fn in os.listdir(tmp):
+            os.remove(os.path.join(tmp, fn))
+        os.rmdir(tmp)
+    out["reference_published"] = {"seqs_per_s": 11968694 / 12.589,
+                                  "note": "64 Emerson repertoires (11,968,694 seqs), Levenshtein d=1, "
+                                          "12.589 s end to end, GPU unrecorded (data/airr_overlap.csv:41)"}
+    return out
+
+
+def copy_probe(nbytes: int = 1 << 30) -> dict:
+    """a plain copy kernel of nbytes on the same GPU: the practical HBM ceiling"""
+    probe = ctypes.CDLL(os.path.join(REPO, "xt-neighbor_amd", "lib", "libxtn_probe.so"))
+    probe.xtn_probe_copy.restype = ctypes.c_double
+    probe.xtn_probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int32]
+    ms = probe.xtn_probe_copy(nbytes, 5)
+    return {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms / 1e3) / 1e9 if ms > 0 else None}
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     runs summarised in profiles/ (calibrated on a copy kernel), or None"""
@@ -199,9 +286,12 @@ def main():
     ap.add_argument("--workload", default="nn-1M-cdr3-lev-d2", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=int, default=300_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="OpenMP threads of the CPU baseline (one GPU's share of the host: 16)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-sets", type=int, default=1000)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-overlap", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--served-index", type=int, default=10_000_000)
     ap.add_argument("--served-queries", type=int, default=64)
@@ -305,8 +395,12 @@ def main():
         result["end_to_end_cli"] = end_to_end(chars, offsets, d, measure)
     if not args.no_served:
         result["served"] = served(args.served_index, args.served_queries, args.served_rate, 5)
+    if not args.no_overlap:
+        result["overlap"] = overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e)
+    result["copy_probe"] = copy_probe()
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(chars, offsets, d, measure, args.cpu_sample)
+        result["cpu_baseline"] = cpu_baseline(chars, offsets, d, measure, args.cpu_sample,
+                                              args.cpu_threads)
     print(json.dumps(result))
 
 

@@ -58,6 +58,9 @@ def lib() -> ctypes.CDLL:
     L.xo_nn_port.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                              ctypes.c_int, ctypes.POINTER(i32p), ctypes.POINTER(Stats)]
     L.xo_nn_port.restype = ctypes.c_int64
+    L.xo_nn_port_mt.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.POINTER(i32p), ctypes.POINTER(Stats)]
+    L.xo_nn_port_mt.restype = ctypes.c_int64
     L.xo_nn_brute.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.POINTER(i32p)]
     L.xo_nn_brute.restype = ctypes.c_int64
@@ -180,6 +183,17 @@ def nn_port(chars, offsets, d: int, measure: int = 0, rounds: int = 1):
     st = Stats()
     n = lib().xo_nn_port(chars.ctypes.data_as(ctypes.c_char_p), _p(offsets), len(offsets) - 1, d,
                          measure, rounds, ctypes.byref(out), ctypes.byref(st))
+    tri = _take_i32(out, 3 * n).reshape(-1, 3)
+    return tri, {name: getattr(st, name) for name, _ in Stats._fields_}
+
+
+def nn_port_mt(chars, offsets, d: int, measure: int = 0, threads: int = 1):
+    """nn_port on `threads` OpenMP threads: (triples int32[n,3], stats dict)"""
+    chars, offsets = _seq_args(chars, offsets)
+    out = ctypes.POINTER(ctypes.c_int32)()
+    st = Stats()
+    n = lib().xo_nn_port_mt(chars.ctypes.data_as(ctypes.c_char_p), _p(offsets), len(offsets) - 1, d,
+                            measure, threads, ctypes.byref(out), ctypes.byref(st))
     tri = _take_i32(out, 3 * n).reshape(-1, 3)
     return tri, {name: getattr(st, name) for name, _ in Stats._fields_}
 

@@ -13,6 +13,7 @@
  */
 #include "xtn_oracle.h"
 
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -448,6 +449,240 @@ int64_t xo_nn_port(const char* chars, const uint32_t* offsets, int n, int distan
 		*stats = st;
 	*out = res;
 	return len;
+}
+
+/* ------------------------------------------------- multi-threaded port -- */
+
+static uint32_t rec_part(const xo_rec* r, int parts) {
+	uint64_t h = ((uint64_t)r->k[0] << 32 | r->k[1]) * 0x9E3779B97F4A7C15ull;
+	h ^= (uint64_t)r->k[2] * 0xC2B2AE3D27D4EB4Full;
+	h ^= h >> 29;
+	return (uint32_t)(h % (uint64_t)parts);
+}
+
+/* every i<j position pair of the equal-key groups of sorted recs[0, n), as
+ * (min id, max id), self pairs included (kernel.cu:157-191, one round) */
+static xo_pair* group_pairs(const xo_rec* recs, int64_t n, int64_t* count) {
+	int64_t c = 0;
+	for (int64_t s = 0; s < n;) {
+		int64_t e = s + 1;
+		while (e < n && rec_key_eq(&recs[s], &recs[e]))
+			e++;
+		c += (e - s) * (e - s - 1) / 2;
+		s = e;
+	}
+	xo_pair* p = (xo_pair*)malloc(sizeof(xo_pair) * (size_t)(c > 0 ? c : 1));
+	int64_t pos = 0;
+	for (int64_t s = 0; s < n;) {
+		int64_t e = s + 1;
+		while (e < n && rec_key_eq(&recs[s], &recs[e]))
+			e++;
+		for (int64_t i = s; i < e; i++)
+			for (int64_t j = i + 1; j < e; j++) {
+				int32_t a = recs[i].v, b = recs[j].v;
+				p[pos].x = a < b ? a : b;
+				p[pos].y = a < b ? b : a;
+				pos++;
+			}
+		s = e;
+	}
+	*count = c;
+	return p;
+}
+
+int64_t xo_nn_port_mt(const char* chars, const uint32_t* offsets, int n, int distance,
+                      int measure, int threads, int32_t** out, xo_stats* stats) {
+	if (threads < 1)
+		threads = 1;
+	const int T = threads;
+	xo_stats st;
+	memset(&st, 0, sizeof(st));
+	uint32_t* seqs = (uint32_t*)malloc(sizeof(uint32_t) * 3 * (size_t)(n > 0 ? n : 1));
+	encode_all(chars, offsets, n, seqs);
+	/* variant counts per sequence range, then generation in place */
+	int64_t* vstart = (int64_t*)calloc((size_t)T + 1, sizeof(int64_t));
+#pragma omp parallel num_threads(T)
+	{
+		const int t = omp_get_thread_num();
+		const int lo = (int)((int64_t)n * t / T), hi = (int)((int64_t)n * (t + 1) / T);
+		int64_t c = 0;
+		for (int i = lo; i < hi; i++)
+			c += xo_combination_len(xo_len_decode(seqs + 3 * (size_t)i), distance);
+		vstart[t + 1] = c;
+	}
+	for (int t = 0; t < T; t++)
+		vstart[t + 1] += vstart[t];
+	const int64_t V = vstart[T];
+	st.n_variants = V;
+	xo_rec* recs = (xo_rec*)malloc(sizeof(xo_rec) * (size_t)(V > 0 ? V : 1));
+	xo_rec* part = (xo_rec*)malloc(sizeof(xo_rec) * (size_t)(V > 0 ? V : 1));
+	int64_t* pcnt = (int64_t*)calloc((size_t)T * T, sizeof(int64_t)); /* [thread][partition] */
+	int64_t* pbase = (int64_t*)calloc((size_t)T * T + 1, sizeof(int64_t));
+	int64_t* pstart = (int64_t*)calloc((size_t)T + 1, sizeof(int64_t));
+	xo_pair** cand = (xo_pair**)calloc((size_t)T, sizeof(xo_pair*));
+	int64_t* ncand = (int64_t*)calloc((size_t)T, sizeof(int64_t));
+	int64_t* xcnt = (int64_t*)calloc((size_t)T * T, sizeof(int64_t)); /* [thread][x range] */
+	int64_t* xbase = (int64_t*)calloc((size_t)T * T + 1, sizeof(int64_t));
+	int64_t* xstart = (int64_t*)calloc((size_t)T + 1, sizeof(int64_t));
+	xo_pair* byx = NULL;
+	int32_t** res = (int32_t**)calloc((size_t)T, sizeof(int32_t*));
+	int64_t* nres = (int64_t*)calloc((size_t)T, sizeof(int64_t));
+	int64_t* nuniq = (int64_t*)calloc((size_t)T, sizeof(int64_t));
+#pragma omp parallel num_threads(T)
+	{
+		const int t = omp_get_thread_num();
+		const int lo = (int)((int64_t)n * t / T), hi = (int)((int64_t)n * (t + 1) / T);
+		/* 1. variants of sequences [lo, hi) (generate_combination.cu) */
+		{
+			int64_t pos = vstart[t];
+			uint32_t* tmp = NULL;
+			int cap = 0;
+			for (int i = lo; i < hi; i++) {
+				const uint32_t* sq = seqs + 3 * (size_t)i;
+				const int c = xo_combination_len(xo_len_decode(sq), distance);
+				if (c > cap) {
+					cap = c;
+					tmp = (uint32_t*)realloc(tmp, sizeof(uint32_t) * 3 * (size_t)c);
+				}
+				xo_expand_keys(sq, distance, tmp, c);
+				for (int k = 0; k < c; k++) {
+					memcpy(recs[pos].k, tmp + 3 * (size_t)k, 3 * sizeof(uint32_t));
+					recs[pos].v = i;
+					pos++;
+				}
+			}
+			free(tmp);
+			for (int64_t r = vstart[t]; r < vstart[t + 1]; r++)
+				pcnt[(size_t)t * T + rec_part(&recs[r], T)]++;
+		}
+#pragma omp barrier
+#pragma omp single
+		{
+			/* partition-major bases: partition p holds thread 0's, then 1's ... records */
+			int64_t acc = 0;
+			for (int p = 0; p < T; p++) {
+				pstart[p] = acc;
+				for (int q = 0; q < T; q++) {
+					pbase[(size_t)q * T + p] = acc;
+					acc += pcnt[(size_t)q * T + p];
+				}
+			}
+			pstart[T] = acc;
+		}
+		/* 2. records into their key partitions */
+		{
+			int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)T);
+			for (int p = 0; p < T; p++)
+				cur[p] = pbase[(size_t)t * T + p];
+			for (int64_t r = vstart[t]; r < vstart[t + 1]; r++)
+				part[cur[rec_part(&recs[r], T)]++] = recs[r];
+			free(cur);
+		}
+#pragma omp barrier
+		/* 3. partition t: stable key sort (cub.cu:95-104), groups, pairs */
+		{
+			xo_rec* pr = part + pstart[t];
+			const int64_t m = pstart[t + 1] - pstart[t];
+			rec_sort(pr, m);
+			cand[t] = group_pairs(pr, m, &ncand[t]);
+			const xo_pair* cp = cand[t];
+			for (int64_t k = 0; k < ncand[t]; k++)
+				xcnt[(size_t)t * T + (int)((int64_t)cp[k].x * T / (n > 0 ? n : 1))]++;
+		}
+#pragma omp barrier
+#pragma omp single
+		{
+			int64_t acc = 0;
+			for (int r = 0; r < T; r++) {
+				xstart[r] = acc;
+				for (int q = 0; q < T; q++) {
+					xbase[(size_t)q * T + r] = acc;
+					acc += xcnt[(size_t)q * T + r];
+				}
+			}
+			xstart[T] = acc;
+			byx = (xo_pair*)malloc(sizeof(xo_pair) * (size_t)(acc > 0 ? acc : 1));
+		}
+		/* 4. pairs into x ranges */
+		{
+			int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * (size_t)T);
+			for (int r = 0; r < T; r++)
+				cur[r] = xbase[(size_t)t * T + r];
+			const xo_pair* cp = cand[t];
+			for (int64_t k = 0; k < ncand[t]; k++)
+				byx[cur[(int)((int64_t)cp[k].x * T / (n > 0 ? n : 1))]++] = cp[k];
+			free(cur);
+			free(cand[t]);
+		}
+#pragma omp barrier
+		/* 5. range t: sort + unique (xtn_inner.cu:138-142), distance and
+		 * filter (kernel.cu:377-392) */
+		{
+			xo_pair* xp = byx + xstart[t];
+			const int64_t m = xstart[t + 1] - xstart[t];
+			qsort(xp, (size_t)m, sizeof(xo_pair), pair_cmp);
+			int64_t cap = 1024, len = 0, u = 0;
+			int32_t* o = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)cap);
+			for (int64_t k = 0; k < m; k++) {
+				if (k > 0 && xp[k].x == xp[k - 1].x && xp[k].y == xp[k - 1].y)
+					continue;
+				u++;
+				const int x = xp[k].x, y = xp[k].y;
+				if (x == y)
+					continue;
+				int d = xo_distance(chars + offsets[x], (int)(offsets[x + 1] - offsets[x]),
+				                    chars + offsets[y], (int)(offsets[y + 1] - offsets[y]), measure);
+				d = (int)(signed char)d;
+				if (d > distance)
+					continue;
+				if (len == cap) {
+					cap *= 2;
+					o = (int32_t*)realloc(o, sizeof(int32_t) * 3 * (size_t)cap);
+				}
+				o[3 * len] = x;
+				o[3 * len + 1] = y;
+				o[3 * len + 2] = d;
+				len++;
+			}
+			res[t] = o;
+			nres[t] = len;
+			nuniq[t] = u;
+		}
+	}
+	int64_t total = 0;
+	for (int t = 0; t < T; t++) {
+		st.n_candidates += ncand[t];
+		st.n_unique += nuniq[t];
+		total += nres[t];
+	}
+	int32_t* all = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(total > 0 ? total : 1));
+	int64_t pos = 0;
+	for (int t = 0; t < T; t++) {
+		memcpy(all + 3 * pos, res[t], sizeof(int32_t) * 3 * (size_t)nres[t]);
+		pos += nres[t];
+		free(res[t]);
+	}
+	st.n_output = total;
+	free(seqs);
+	free(vstart);
+	free(recs);
+	free(part);
+	free(pcnt);
+	free(pbase);
+	free(pstart);
+	free(cand);
+	free(ncand);
+	free(xcnt);
+	free(xbase);
+	free(xstart);
+	free(byx);
+	free(res);
+	free(nres);
+	free(nuniq);
+	if (stats)
+		*stats = st;
+	*out = all;
+	return total;
 }
 
 int64_t xo_nn_brute(const char* chars, const uint32_t* offsets, int n, int distance,

@@ -25,6 +25,8 @@
  *                          tiny strings)
  *   xo_nn_port          -> xo_nn_brute (independent O(n^2) definition) plus
  *                          the reference fixture data/dummy_input.txt
+ *   xo_nn_port_mt       -> xo_nn_port (triples and counters) at several
+ *                          thread counts
  *   xo_overlap_port     -> test_overlap_init.cu, test_stream4_overlap.cu,
  *                          data/dummy_input_airr.tsv expected matrix, and the
  *                          closed form R^T (A + I) R on duplicate-free input
@@ -73,6 +75,14 @@ int xo_distance(const char* a, int la, const char* b, int lb, int measure);
  * *out receives 3 int32 per output pair (i, j, dist), sorted by (i, j). */
 int64_t xo_nn_port(const char* chars, const uint32_t* offsets, int n, int distance,
                    int measure, int rounds, int32_t** out, xo_stats* stats);
+/* xo_nn_port on `threads` OpenMP threads (the timed multi-core CPU baseline):
+ * variants generated per sequence range, records hash-partitioned by key so
+ * that a key group stays in one partition, each partition sorted and paired
+ * by one thread, pairs partitioned by x range, sorted, deduplicated and
+ * verified per range, ranges concatenated in order. Same output and counters
+ * as xo_nn_port with one round. */
+int64_t xo_nn_port_mt(const char* chars, const uint32_t* offsets, int n, int distance,
+                      int measure, int threads, int32_t** out, xo_stats* stats);
 /* NN search by definition: every i<j with dist <= distance. */
 int64_t xo_nn_brute(const char* chars, const uint32_t* offsets, int n, int distance,
                     int measure, int32_t** out);

@@ -228,3 +228,17 @@ def test_neighbors_of_matches_brute():
             want = sorted([(int(b), int(c)) for a, b, c in tri if a == row] +
                           [(int(a), int(c)) for a, b, c in tri if b == row])
             assert oracle.neighbors_of(chars, offsets, row, d, measure).tolist() == [list(x) for x in want]
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_port_mt_equals_port(threads):
+    rng = np.random.default_rng(threads)
+    seqs = clustered_seqs(rng, 1500, 1, 24, "ACDEFG", edits=2) + ["CASS"] * 7
+    chars, offsets = xtn.encode(seqs)
+    for d, measure in [(1, 0), (2, 0), (1, 1)]:
+        want, wst = oracle.nn_port(chars, offsets, d, measure)
+        got, gst = oracle.nn_port_mt(chars, offsets, d, measure, threads)
+        assert np.array_equal(got, want)
+        assert gst == wst
+    empty, _ = oracle.nn_port_mt(*xtn.encode(["CASS"]), 1, 0, threads)
+    assert len(empty) == 0

@@ -16,6 +16,7 @@
 #include "cdr3_synth.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -340,4 +341,25 @@ int64_t xs_generate_queries(uint64_t seed, const char* idx_chars, const uint32_t
 	}
 	strset_free(&ss);
 	return total;
+}
+
+/* ------------------------------------------------------------ CSV out -- */
+
+int32_t xs_write_csv(const char* path, const char* chars, const uint32_t* offsets, int32_t n,
+                     const int32_t* freq) {
+	FILE* f = fopen(path, "wb");
+	if (!f)
+		return -1;
+	static char buf[1 << 20];
+	setvbuf(f, buf, _IOFBF, sizeof(buf));
+	fputs(freq ? "cdr3,count\n" : "cdr3\n", f);
+	for (int32_t i = 0; i < n; i++) {
+		fwrite(chars + offsets[i], 1, offsets[i + 1] - offsets[i], f);
+		if (freq)
+			fprintf(f, ",%d\n", freq[i]);
+		else
+			fputc('\n', f);
+	}
+	const int bad = ferror(f);
+	return (fclose(f) != 0 || bad) ? -1 : 0;
 }

@@ -25,6 +25,11 @@ int64_t xs_generate_queries(uint64_t seed, const char* idx_chars, const uint32_t
                             int32_t n_index, int32_t n, double mu, double sigma, int32_t lmin,
                             int32_t lmax, double planted_frac, char* chars, int64_t cap_chars,
                             uint32_t* offsets);
+/* writes the n sequences as the CLI's input CSV: a header line, then one
+   "sequence\n" per row, or "sequence,frequency\n" when freq is not NULL.
+   Returns 0, or -1 when the file cannot be written. */
+int32_t xs_write_csv(const char* path, const char* chars, const uint32_t* offsets, int32_t n,
+                     const int32_t* freq);
 #ifdef __cplusplus
 }
 #endif

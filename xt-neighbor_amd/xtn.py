@@ -171,6 +171,8 @@ def load_synth(path: str = SYNTH_PATH) -> ctypes.CDLL:
                                       ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_double, c_char_pp, ctypes.c_int64, u32p]
     s.xs_generate_queries.restype = ctypes.c_int64
+    s.xs_write_csv.argtypes = [ctypes.c_char_p, c_char_pp, u32p, ctypes.c_int32, i32p]
+    s.xs_write_csv.restype = ctypes.c_int32
     _synth = s
     return s
 
@@ -244,6 +246,20 @@ def synth_queries(seed: int, idx_chars: np.ndarray, idx_offsets: np.ndarray, n: 
     if total < 0:
         raise RuntimeError(f"xs_generate_queries failed ({total})")
     return chars[:total].copy(), offsets
+
+
+def write_csv(path: str, chars: np.ndarray, offsets: np.ndarray, freq: np.ndarray | None = None) -> None:
+    """the sequences (and frequencies) as the CLI's input CSV"""
+    chars = np.ascontiguousarray(chars, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+    fp = None
+    if freq is not None:
+        freq = np.ascontiguousarray(freq, dtype=np.int32)
+        fp = _i32p(freq)
+    rc = load_synth().xs_write_csv(path.encode(), chars.ctypes.data_as(ctypes.c_char_p), _u32p(offsets),
+                                   len(offsets) - 1, fp)
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
 
 
 def reps_from_sizes(sizes: Sequence[int]) -> np.ndarray:
@@ -356,11 +372,12 @@ class Engine:
         freq = np.asarray(freq, dtype=np.int32)
         rep = np.asarray(rep, dtype=np.int32)
         n = len(freq)
-        info = (SeqInfo * max(n, 1))()
-        for i in range(n):
-            info[i].frequency = int(freq[i])
-            info[i].repertoire = int(rep[i])
-            info[i].original_index = i
+        rows = np.zeros(max(n, 1), dtype=[("frequency", "<i4"), ("repertoire", "<i4"),
+                                           ("original_index", "<i4")])
+        rows["frequency"][:n] = freq
+        rows["repertoire"][:n] = rep
+        rows["original_index"][:n] = np.arange(n, dtype=np.int32)
+        info = rows.ctypes.data_as(ctypes.POINTER(SeqInfo))
         parts = []
         calls = [0]
 
