@@ -266,6 +266,11 @@ def copy_probe(nbytes: int = 1 << 30) -> dict:
     return {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms / 1e3) / 1e9 if ms > 0 else None}
 
 
+def progress(msg: str) -> None:
+    """one line per finished leg on stderr (stdout carries only the JSON line)"""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     runs summarised in profiles/ (calibrated on a copy kernel), or None"""
@@ -344,6 +349,7 @@ def main():
         eng.nn_resident()
     hip_sync()
     profiled_ms = (time.perf_counter() - tp0) / args.steps * 1e3
+    progress(f"{args.workload}: {step_s * 1e3:.3f} ms per step")
     kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "region", "bucket", "runs", "pairs_small", "pairs_big",
                "sort_seg_hist", "sort_seg_scatter", "sort_pairs_hist", "sort_pairs_scatter", "unique", "verify",
                "emit", "pack", "scan", "gen_count"]
@@ -390,17 +396,23 @@ def main():
     }
     if not args.no_batched:
         result["batched"] = batched_vs_plain(eng, offsets, args.batched_sets, seed)
+        progress("batched vs plain done")
     eng.close()
     if not args.no_e2e:
         result["end_to_end_cli"] = end_to_end(chars, offsets, d, measure)
+        progress("CLI end to end done")
     if not args.no_served:
         result["served"] = served(args.served_index, args.served_queries, args.served_rate, 5)
+        progress("served done")
     if not args.no_overlap:
         result["overlap"] = overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e)
+        progress("overlap done")
     result["copy_probe"] = copy_probe()
+    progress("copy probe done")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(chars, offsets, d, measure, args.cpu_sample,
                                               args.cpu_threads)
+        progress("CPU baseline done")
     print(json.dumps(result))
 
 

