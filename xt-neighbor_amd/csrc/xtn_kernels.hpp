@@ -681,18 +681,21 @@ struct RegionArgs {
 	int32_t fine_bits;      // B - 8
 };
 
-__global__ __launch_bounds__(RS_NT) void k_region_scatter(RegionArgs a) {
+template <int NT = RS_NT>
+__global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
+	constexpr int IT = RS_TILE / NT;   // keys per thread
+	constexpr int SEG = IT * WAVE;     // keys per wave
 	__shared__ uint64_t s_keys[RS_TILE];
-	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+	__shared__ uint32_t s_cnt[NT / WAVE][RS_BINS];
 	__shared__ uint32_t s_dpre[RS_BINS + 1];
 	__shared__ uint32_t s_gbase[RS_BINS];
-	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+	__shared__ uint32_t s_scan[NT / WAVE + 1];
 	__shared__ uint32_t s_ts[RS_BINS + 1];
 
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
-	for (int i = t; i <= RS_BINS; i += RS_NT)
+	for (int i = t; i <= RS_BINS; i += NT)
 		s_ts[i] = a.ctl[RS_BINS + 1 + i];
 	__syncthreads();
 	const uint32_t ntiles = s_ts[RS_BINS];
@@ -712,19 +715,19 @@ __global__ __launch_bounds__(RS_NT) void k_region_scatter(RegionArgs a) {
 	const int64_t base = cbeg + (int64_t)(tile - s_ts[c]) * RS_TILE;
 	const int tile_n = (int)(cend - base < RS_TILE ? cend - base : RS_TILE);  // >= 1
 
-	const int64_t seg = base + (int64_t)w * RS_SEG;
-	const int wrem = tile_n - w * RS_SEG;
-	const int w_valid = wrem <= 0 ? 0 : (wrem < RS_SEG ? wrem : RS_SEG);
-	uint64_t k[RS_IT];
+	const int64_t seg = base + (int64_t)w * SEG;
+	const int wrem = tile_n - w * SEG;
+	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? wrem : SEG);
+	uint64_t k[IT];
 #pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
+	for (int r = 0; r < IT; ++r) {
 		const bool valid = r * WAVE + lane < w_valid;
 		k[r] = valid ? a.in[seg + (int64_t)r * WAVE + lane] : 0ull;
 	}
-	uint32_t lp[RS_IT];
-	rank_tile(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);
+	uint32_t lp[IT];
+	rank_tile_g<NT, IT>(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);
 #pragma unroll
-	for (int r = 0; r < RS_IT; ++r)
+	for (int r = 0; r < IT; ++r)
 		if (lp[r] != 0xFFFFFFFFu)
 			s_keys[lp[r]] = k[r];
 	if (t == 0)
@@ -732,12 +735,12 @@ __global__ __launch_bounds__(RS_NT) void k_region_scatter(RegionArgs a) {
 	__syncthreads();
 	// one cursor claim per digit run of the tile
 	const uint32_t fmask = (1u << a.fine_bits) - 1u;
-	{
+	if (t < RS_BINS) {
 		const uint32_t cnt = s_dpre[t + 1] - s_dpre[t];
 		s_gbase[t] = cnt ? atomicAdd(&a.count[(c << a.fine_bits) | ((uint32_t)t & fmask)], cnt) : 0u;
 	}
 	__syncthreads();
-	for (int i = t; i < tile_n; i += RS_NT) {
+	for (int i = t; i < tile_n; i += NT) {
 		const uint64_t key = s_keys[i];
 		const uint32_t d = (uint32_t)(key >> a.shift) & 0xFFu;
 		const uint32_t slot = s_gbase[d] + ((uint32_t)i - s_dpre[d]);

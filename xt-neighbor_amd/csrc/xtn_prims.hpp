@@ -320,20 +320,23 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t i, int64_t ntiles) {
 	return xcd * q + (xcd < r ? xcd : r) + local;
 }
 
-// Stable rank of a tile of up to RS_NT * IT keys by the 8-bit digit at
-// `shift`. Wave w holds the keys of tile positions [w*IT*64, (w+1)*IT*64),
+// Stable rank of a tile of up to NT * IT keys by the 8-bit digit at `shift`
+// (NT threads, NT / 64 waves). Wave w holds the keys of tile positions
+// [w*IT*64, (w+1)*IT*64),
 // round r lane l holding position w*IT*64 + r*64 + l; only the first w_valid
 // positions of a wave are keys. On return lp[r] is the key's position in the
 // tile sorted by digit (stable), or 0xFFFFFFFF when invalid, and s_dpre[d]
 // the first position of digit d. Every thread must call it.
-template <int IT>
-__device__ __forceinline__ void rank_tile_n(const uint64_t (&k)[IT], int w_valid, int shift,
+template <int NT, int IT>
+__device__ __forceinline__ void rank_tile_g(const uint64_t (&k)[IT], int w_valid, int shift,
                                             uint32_t (&lp)[IT], uint32_t (*s_cnt)[RS_BINS],
                                             uint32_t* s_dpre, uint32_t* s_scan) {
+	static_assert(NT % WAVE == 0 && NT >= RS_BINS, "one thread per digit in the prefix step");
+	constexpr int NW = NT / WAVE;
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
-	for (int i = t; i < RS_WAVES * RS_BINS; i += RS_NT)
+	for (int i = t; i < NW * RS_BINS; i += NT)
 		(&s_cnt[0][0])[i] = 0;
 	__syncthreads();
 	const uint64_t lt = lanemask_lt();
@@ -355,20 +358,22 @@ __device__ __forceinline__ void rank_tile_n(const uint64_t (&k)[IT], int w_valid
 	__syncthreads();
 	{
 		const int d = t;
-		uint32_t c[RS_WAVES];
+		uint32_t c[NW];
 		uint32_t tot = 0;
 #pragma unroll
-		for (int ww = 0; ww < RS_WAVES; ++ww) {
-			c[ww] = s_cnt[ww][d];
+		for (int ww = 0; ww < NW; ++ww) {
+			c[ww] = d < RS_BINS ? s_cnt[ww][d] : 0u;
 			tot += c[ww];
 		}
-		const uint32_t ex = block_excl_scan<RS_NT>(tot, s_scan, (uint32_t*)nullptr);
-		s_dpre[d] = ex;
-		uint32_t run = ex;
+		const uint32_t ex = block_excl_scan<NT>(tot, s_scan, (uint32_t*)nullptr);
+		if (d < RS_BINS) {
+			s_dpre[d] = ex;
+			uint32_t run = ex;
 #pragma unroll
-		for (int ww = 0; ww < RS_WAVES; ++ww) {
-			s_cnt[ww][d] = run;
-			run += c[ww];
+			for (int ww = 0; ww < NW; ++ww) {
+				s_cnt[ww][d] = run;
+				run += c[ww];
+			}
 		}
 	}
 	__syncthreads();
@@ -377,49 +382,59 @@ __device__ __forceinline__ void rank_tile_n(const uint64_t (&k)[IT], int w_valid
 		lp[r] = rk[r] == 0xFFFFFFFFu ? 0xFFFFFFFFu : s_cnt[w][rk[r] & 0xFFu] + (rk[r] >> 8);
 }
 
+template <int IT>
+__device__ __forceinline__ void rank_tile_n(const uint64_t (&k)[IT], int w_valid, int shift,
+                                            uint32_t (&lp)[IT], uint32_t (*s_cnt)[RS_BINS],
+                                            uint32_t* s_dpre, uint32_t* s_scan) {
+	rank_tile_g<RS_NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+}
+
 __device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_valid, int shift,
                                           uint32_t (&lp)[RS_IT], uint32_t (*s_cnt)[RS_BINS],
                                           uint32_t* s_dpre, uint32_t* s_scan) {
 	rank_tile_n<RS_IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 }
 
-template <bool KV>
-__global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
+template <bool KV, int NT = RS_NT>
+__global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
                                                          uint64_t* __restrict__ keys_out,
                                                          uint32_t* __restrict__ vals_out, int64_t n,
                                                          int shift, const uint32_t* __restrict__ offs,
                                                          int64_t ntiles) {
+	constexpr int IT = RS_TILE / NT;   // keys per thread
+	constexpr int SEG = IT * WAVE;     // keys per wave
 	__shared__ uint64_t s_keys[RS_TILE];
 	__shared__ uint32_t s_vals[KV ? RS_TILE : 1];
-	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+	__shared__ uint32_t s_cnt[NT / WAVE][RS_BINS];
 	__shared__ uint32_t s_dpre[RS_BINS];
 	__shared__ uint32_t s_gbase[RS_BINS];
-	__shared__ uint32_t s_scan[RS_NT / WAVE + 1];
+	__shared__ uint32_t s_scan[NT / WAVE + 1];
 
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
 	const int64_t tile = xcd_tile(blockIdx.x, ntiles);
 	const int64_t base = tile * RS_TILE;
-	const int64_t seg = base + (int64_t)w * RS_SEG;
+	const int64_t seg = base + (int64_t)w * SEG;
 	const int64_t wrem = n - seg;
-	const int w_valid = wrem <= 0 ? 0 : (wrem < RS_SEG ? (int)wrem : RS_SEG);
-	uint64_t k[RS_IT];
-	uint32_t v[RS_IT];
+	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? (int)wrem : SEG);
+	uint64_t k[IT];
+	uint32_t v[IT];
 #pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
+	for (int r = 0; r < IT; ++r) {
 		const bool valid = r * WAVE + lane < w_valid;
 		const int64_t idx = seg + (int64_t)r * WAVE + lane;
 		k[r] = valid ? keys_in[idx] : 0ull;
 		if (KV)
 			v[r] = valid ? vals_in[idx] : 0u;
 	}
-	s_gbase[t] = offs[tile * RS_BINS + t];
-	uint32_t lp[RS_IT];
-	rank_tile(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+	if (t < RS_BINS)
+		s_gbase[t] = offs[tile * RS_BINS + t];
+	uint32_t lp[IT];
+	rank_tile_g<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 #pragma unroll
-	for (int r = 0; r < RS_IT; ++r) {
+	for (int r = 0; r < IT; ++r) {
 		if (lp[r] != 0xFFFFFFFFu) {
 			s_keys[lp[r]] = k[r];
 			if (KV)
@@ -430,7 +445,7 @@ __global__ __launch_bounds__(RS_NT) void k_radix_scatter(const uint64_t* __restr
 
 	const int64_t rem = n - base;
 	const int tile_n = rem < RS_TILE ? (int)rem : RS_TILE;
-	for (int i = t; i < tile_n; i += RS_NT) {
+	for (int i = t; i < tile_n; i += NT) {
 		const uint64_t key = s_keys[i];
 		const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
 		const uint32_t g = s_gbase[d] + ((uint32_t)i - s_dpre[d]);
