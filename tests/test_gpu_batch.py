@@ -85,3 +85,32 @@ def test_sets_over_a_row_subrange(engine):
     got = engine.nn_sets(so, batched=True)
     same(got, engine.nn_sets(so, batched=False))
     same(got, per_set_brute(chars, offsets, so, 1, 0))
+
+
+def _device_free_bytes():
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value
+
+
+def test_close_releases_device_memory():
+    # every workspace of a handle (sort, bucket, set and output buffers) is
+    # freed by xtn_close: repeated open / run / close cycles leave the free
+    # device memory where it was
+    rng = np.random.default_rng(11)
+    seqs, offs = make_sets(rng, [int(x) for x in rng.integers(50, 400, 40)])
+    chars, offsets = xtn.encode(seqs)
+
+    def cycle():
+        with xtn.Engine(distance=1) as e:
+            e.set_sequences(chars=chars, offsets=offsets)
+            e.nn()
+            e.nn_sets(offs, batched=True)
+
+    cycle()
+    free0 = _device_free_bytes()
+    for _ in range(3):
+        cycle()
+    assert free0 - _device_free_bytes() < (16 << 20)

@@ -3,8 +3,8 @@
 // Data layout in HBM (one key-space chunk):
 //   pk[2*i], pk[2*i+1]   18-residue prefix of sequence i packed as a 90-bit
 //                        integer (residue 0 most significant, 5 bits each,
-//                        code = c - '@'), prefix length in bits 56..63 of the
-//                        high word.
+//                        code = c - '@'), the query set in bits 26..55 and
+//                        the prefix length in bits 56..63 of the high word.
 //   voff[i]              first deletion record of sequence i (exclusive scan
 //                        of the per-sequence variant counts), voff[n] = V.
 //   rec[r]               one deletion record: 64-bit word, the high 64-idb
@@ -27,6 +27,9 @@
 namespace xtn {
 
 constexpr int MAX_PREFIX = 18;      // xtn.h:8 MAX_COMPRESSED_LENGTH
+constexpr int PK_SET_SHIFT = 26;    // packed high word: 26 residue bits, then the set id
+constexpr uint32_t PK_SET_MASK = (1u << 30) - 1u;
+constexpr uint64_t PK_HI_MASK = (1ull << 56) - 1ull;  // residues + set, without the length byte
 constexpr int HAMMING_SENTINEL = 77;  // kernel.cu:292
 constexpr int SMALL_RUN_MAX = 32;   // runs longer than this use the flattened pair kernel
 
@@ -61,9 +64,15 @@ __device__ __forceinline__ uint64_t variant_count(int L, int d) {
 
 // one thread per sequence: pack the 18-prefix, count its deletion variants
 // (kernel.cu:33-57 counts the same sum of binomials)
+// pk[2i], pk[2i+1]: the 90-bit prefix (residue 0 most significant, 5 bits
+// each), then in the high word bits 26..55 the query set of the id (batched
+// form, set_of non-null; those bits lie above every residue slot, so they
+// ride through the deletions into the variant hash) and bits 56..63 the
+// prefix length.
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
                                               const uint32_t* __restrict__ offs,
                                               const uint32_t* __restrict__ seq_map, uint32_t row_base,
+                                              const uint32_t* __restrict__ set_of,
                                               int32_t n, int distance, uint64_t* __restrict__ pk,
                                               uint64_t* __restrict__ vcount) {
 	const int32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -85,6 +94,8 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
 				hi |= code >> (64 - sh);
 		}
 	}
+	if (set_of)
+		hi |= (uint64_t)(set_of[i] & PK_SET_MASK) << PK_SET_SHIFT;
 	pk[2 * (int64_t)i] = lo;
 	pk[2 * (int64_t)i + 1] = hi | ((uint64_t)L << 56);
 	vcount[i] = variant_count(L, distance);
@@ -92,23 +103,17 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
 
 // ---------------------------------------------------------- generation --
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-	x ^= x >> 32;
-	x *= 0xD6E8FEB86659FD93ull;
-	x ^= x >> 32;
-	x *= 0xD6E8FEB86659FD93ull;
-	x ^= x >> 32;
-	return x;
-}
-
-// 90-bit packed variant -> 64-bit key. The 26 high bits enter through an odd
+// 90-bit packed variant -> 64-bit key. The high bits enter through an odd
 // multiplier (injective), then one xorshift-multiply-xorshift round spreads
-// every input bit into the high bits the records are grouped by. Two
-// quarter-rate multiplies fewer than a full mix64 of the folded value.
+// every input bit into the high bits the records are grouped by; one 64x64
+// multiply after the fold, half the quarter-rate multiplies of a two-round
+// finaliser. With
+// `wide` the high word also carries the query set (bits 26..55).
+template <bool wide>
 __device__ __forceinline__ uint64_t hash_variant(u128 v) {
 	const uint64_t lo = (uint64_t)v;
-	const uint32_t hi = (uint32_t)(v >> 64);  // 26 bits
-	uint64_t x = lo ^ ((uint64_t)hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull;
+	const uint64_t hi = wide ? (uint64_t)(v >> 64) : (uint64_t)(uint32_t)(v >> 64);  // 26 bits unless wide
+	uint64_t x = lo ^ (hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull;
 	x ^= x >> 32;
 	x *= 0xD6E8FEB86659FD93ull;
 	x ^= x >> 29;
@@ -302,7 +307,7 @@ struct GenArgs {
 	int64_t cap;
 	unsigned long long* part_hist;  // GEN_HIST: 2^part_bits counters
 	unsigned long long* append;     // GEN_APPEND cursor
-	const uint32_t* set_of;         // nullable: query set of each id, salted into the key
+	const uint32_t* set_of;         // non-null in the batched form: the set ids k_pack put in pk
 	uint32_t* digit_hist;           // GEN_DIRECT, nullable: per-tile counts of the
 	int64_t ntiles;                 // 8-bit digit at digit_shift, one row of
 	int32_t digit_shift;            // RS_BINS per tile as k_radix_hist writes them
@@ -422,7 +427,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			const uint32_t k = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_vrel[idx]));
 			const uint64_t lo64 = a.pk[2 * sid];
 			const uint64_t hiw = a.pk[2 * sid + 1];
-			const u128 p = ((u128)(hiw & 0x3FFFFFFull) << 64) | (u128)lo64;
+			const u128 p = ((u128)(hiw & PK_HI_MASK) << 64) | (u128)lo64;
 			u128 v;
 			if (small_d) {  // uniform: branch-free unranking
 				const uint32_t e = s_u.del[k];
@@ -436,9 +441,8 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 				const int m = a.distance < L ? a.distance : L;
 				v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
 			}
-			uint64_t hv = hash_variant(v);
-			if (a.set_of)  // variants of different query sets get different keys
-				hv ^= mix64(((uint64_t)a.set_of[sid] + 1ull) * 0x9E3779B97F4A7C15ull);
+			// the set bits make variants of different query sets different keys
+			const uint64_t hv = a.set_of ? hash_variant<true>(v) : hash_variant<false>(v);
 			rec = (hv & ~idmask) | ((uint64_t)sid & idmask);
 		}
 		if (MODE == GEN_DIRECT) {
@@ -1056,8 +1060,7 @@ __device__ __forceinline__ int seq_distance(const uint8_t* a, int la, const uint
 	case 5: return lev_band<5>(a, la, b, lb);
 	case 6: return lev_band<6>(a, la, b, lb);
 	case 7: return lev_band<7>(a, la, b, lb);
-	case 8: return lev_band<8>(a, la, b, lb);
-	default: return lev_band_rt(a, la, b, lb, distance);
+	default: return lev_band<8>(a, la, b, lb);  // callers take the wide band above 8
 	}
 }
 
@@ -1147,10 +1150,17 @@ __device__ __forceinline__ int hamming_reg(const uint32_t (&A)[REG_WORDS], int l
 	return d;
 }
 
-// seq_distance with the register path when both strings fit it
+// seq_distance with the register path when both strings fit it. WIDE: the
+// Levenshtein band of a distance above 8, whose private band array
+// (lev_band_rt) only the kernels instantiated for such distances contain, so
+// that the others need no scratch memory (the runtime reserves scratch for
+// every resident wave of a kernel that uses it).
+template <bool WIDE>
 __device__ __forceinline__ int pair_distance(const uint8_t* a, int la, const uint8_t* b, int lb,
                                              int distance, int measure) {
-	if (la > REG_LEN || lb > REG_LEN || (measure != 1 && distance > 8))
+	if (WIDE)
+		return lev_band_rt(a, la, b, lb, distance);
+	if (la > REG_LEN || lb > REG_LEN)
 		return seq_distance(a, la, b, lb, distance, measure);
 	uint32_t A[REG_WORDS], B[REG_WORDS];
 	load_residues(a, la, A);
@@ -1232,6 +1242,7 @@ __global__ __launch_bounds__(SEL_NT) void k_uniq_write(const uint64_t* __restric
 // latency-bound: the more pairs in flight the better); *n_dev of them (the
 // scan total of the previous step), the grid covers the largest possible
 // count. Tiles of SEL_NT pairs.
+template <bool WIDE>
 __global__ __launch_bounds__(SEL_NT) void k_verify_count(const uint64_t* __restrict__ upairs,
                                                          const uint32_t* __restrict__ n_dev, int idb,
                                                          const uint8_t* __restrict__ chars,
@@ -1261,7 +1272,7 @@ __global__ __launch_bounds__(SEL_NT) void k_verify_count(const uint64_t* __restr
 				const uint32_t oa = seq_map ? seq_map[a] : row_base + a;
 				const uint32_t ob = seq_map ? seq_map[b] : row_base + b;
 				const uint32_t sa = offs[oa], sb = offs[ob];
-				const int d = pair_distance(chars + sa, (int)(offs[oa + 1] - sa), chars + sb,
+				const int d = pair_distance<WIDE>(chars + sa, (int)(offs[oa + 1] - sa), chars + sb,
 				                            (int)(offs[ob + 1] - sb), distance, measure);
 				if (d <= distance) {
 					dv = (int8_t)d;
@@ -1313,6 +1324,25 @@ __global__ __launch_bounds__(256) void k_emit_nn(const uint64_t* __restrict__ pa
 	const uint32_t o = pos[q];
 	out_pairs[o] = make_int2((int)(p >> idb), (int)(p & ((1ull << idb) - 1ull)));
 	out_dist[o] = dist[q];
+}
+
+// set_of[r] = the query set holding row r0 + r: the largest k with
+// offs[k] <= r0 + r (empty sets share their offset with the next set)
+__global__ __launch_bounds__(256) void k_set_of(const uint32_t* __restrict__ offs, int32_t n_sets,
+                                                uint32_t r0, uint32_t n, uint32_t* __restrict__ set_of) {
+	const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+	if (r >= n)
+		return;
+	const uint32_t row = r0 + r;
+	int lo = 0, hi = n_sets - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (offs[mid] <= row)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	set_of[r] = (uint32_t)lo;
 }
 
 __global__ void k_fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {

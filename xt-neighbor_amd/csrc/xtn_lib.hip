@@ -87,7 +87,7 @@ struct xtn_handle {
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAcc> prof;
 	std::vector<PendingEvent> spans;  // stage timing, always on
-	DBuf bstart, ovf, set_of;
+	DBuf bstart, ovf, set_of, set_offs;
 	int64_t last_pairs = 0;
 	// host copy of the prefix lengths, and per-row prefix sums of the
 	// variant counts for one distance: V of any row range without a readback
@@ -528,7 +528,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		StageScope sg(h, "generate");
 		ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
 		k_pack<<<grid_for(nseq, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
-		                                            seq_map, row_base, nseq, d, h->pk.as<uint64_t>(),
+		                                            seq_map, row_base, set_of, nseq, d, h->pk.as<uint64_t>(),
 		                                            h->voff.as<uint64_t>());
 		HIP_LAUNCH_CHECK(h);
 	}
@@ -827,7 +827,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		}
 		{
 			ProfScope ps(h, "verify", 0.0);
-			k_verify_count<<<(unsigned)nv, SEL_NT, 0, s>>>(
+			auto verify = measure == 0 && d > 8 ? k_verify_count<true> : k_verify_count<false>;
+			verify<<<(unsigned)nv, SEL_NT, 0, s>>>(
 			    upairs, ucnt + nt, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), seq_map, row_base,
 			    set_of, d, measure, h->dist.as<int8_t>(), kcnt);
 			HIP_LAUNCH_CHECK(h);
@@ -1088,7 +1089,8 @@ void xtn_close(xtn_handle* h) {
 	                &h->pairs_alt, &h->flags, &h->pos, &h->dist, &h->out_pairs, &h->out_dist,
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
 	                &h->ov_vals_alt, &h->ov_info, &h->ov_leader, &h->ov_ustart, &h->ov_umap,
-	                &h->ov_cnt, &h->ov_matrix, &h->ov_present};
+	                &h->ov_cnt, &h->ov_matrix, &h->ov_present, &h->bstart, &h->ovf, &h->set_of,
+	                &h->set_offs};
 	for (DBuf* b : bufs)
 		release(*b);
 	for (auto& pe : h->pending) {
@@ -1269,6 +1271,8 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 	for (int32_t k = 0; k < n_sets; ++k)
 		if (set_offsets[k + 1] < set_offsets[k])
 			return fail(h, XTN_ERR_ARG, "set offsets must be non-decreasing");
+	if (n_sets > (1 << 30))  // set ids are packed into 30 bits
+		return fail(h, XTN_ERR_ARG, "too many query sets (%d)", n_sets);
 	if (n_sets > 0 && set_offsets[n_sets] > (uint32_t)h->n)
 		return fail(h, XTN_ERR_ARG, "query sets exceed the %d rows", h->n);
 	HIP_TRY(h, hipSetDevice(h->cfg.device));
@@ -1296,14 +1300,21 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 	std::vector<int8_t> hd;
 	if (batched) {
 		const uint32_t r0 = set_offsets[0], r1 = set_offsets[n_sets];
-		std::vector<uint32_t> set_of((size_t)std::max<uint32_t>(r1 - r0, 1));
-		for (int32_t k = 0; k < n_sets; ++k)
-			for (uint32_t r = set_offsets[k]; r < set_offsets[k + 1]; ++r)
-				set_of[r - r0] = (uint32_t)k;
-		TRY(ensure(h, h->set_of, sizeof(uint32_t) * set_of.size()));
-		HIP_TRY(h, hipMemcpyAsync(h->set_of.p, set_of.data(), sizeof(uint32_t) * set_of.size(),
+		// set of every row, on the device from the n_sets + 1 offsets
+		TRY(ensure(h, h->set_offs, sizeof(uint32_t) * ((size_t)n_sets + 1)));
+		TRY(ensure(h, h->set_of, sizeof(uint32_t) * (size_t)std::max<uint32_t>(r1 - r0, 1)));
+		HIP_TRY(h, hipMemcpyAsync(h->set_offs.p, set_offsets, sizeof(uint32_t) * ((size_t)n_sets + 1),
 		                          hipMemcpyHostToDevice, h->stream));
+		if (r1 > r0) {
+			k_set_of<<<grid_for(r1 - r0, 256), 256, 0, h->stream>>>(h->set_offs.as<uint32_t>(), n_sets, r0,
+			                                                       r1 - r0, h->set_of.as<uint32_t>());
+			HIP_LAUNCH_CHECK(h);
+		}
 		TRY(nn_pipeline(h, (int32_t)(r1 - r0), nullptr, r0, h->set_of.as<uint32_t>()));
+		// the query set holding a row, as k_set_of finds it
+		auto set_of = [&](uint32_t row) -> uint32_t {
+			return (uint32_t)(std::upper_bound(set_offsets, set_offsets + n_sets, row) - set_offsets - 1);
+		};
 		accumulate();
 		if (callback && h->out_len > 0) {
 			TRY(copy_output(h, hp, hd));
@@ -1311,10 +1322,11 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 			int64_t i = 0;
 			const int64_t n = h->out_len;
 			while (i < n) {
-				const uint32_t k = set_of[(uint32_t)hp[i].x];
+				const uint32_t k = set_of(r0 + (uint32_t)hp[i].x);
 				const int32_t base = (int32_t)(set_offsets[k] - r0);
+				const int32_t end = (int32_t)(set_offsets[k + 1] - r0);
 				int64_t j = i;
-				while (j < n && set_of[(uint32_t)hp[j].x] == k) {
+				while (j < n && hp[j].x < end) {
 					hp[j].x -= base;
 					hp[j].y -= base;
 					++j;
@@ -1457,8 +1469,8 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	TRY(ensure(w, w->pk, sizeof(uint64_t) * 2 * (size_t)n));
 	TRY(ensure(w, w->voff, sizeof(uint64_t) * ((size_t)n + 1)));
 	TRY(ensure(w, w->ctrs, sizeof(RunCtrs)));
-	k_pack<<<grid_for(n, 256), 256, 0, s>>>(w->chars.as<uint8_t>(), w->offs.as<uint32_t>(), nullptr, 0, n, d,
-	                                         w->pk.as<uint64_t>(), w->voff.as<uint64_t>());
+	k_pack<<<grid_for(n, 256), 256, 0, s>>>(w->chars.as<uint8_t>(), w->offs.as<uint32_t>(), nullptr, 0, nullptr,
+	                                         n, d, w->pk.as<uint64_t>(), w->voff.as<uint64_t>());
 	HIP_LAUNCH_CHECK(w);
 	TRY(scan_excl<uint64_t>(w, w->voff.as<uint64_t>(), w->voff.as<uint64_t>(), n, w->voff.as<uint64_t>() + n));
 	const int64_t V = rq.variants;  // the same closed form the device scan computes
@@ -1499,10 +1511,11 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	TRY(ensure(w, w->flags, sizeof(uint32_t) * (size_t)np));
 	TRY(ensure(w, w->pos, sizeof(uint32_t) * ((size_t)np + 1)));
 	TRY(ensure(w, w->dist, (size_t)np));
-	k_verify_cross<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->chars.as<uint8_t>(),
-	                                                  w->offs.as<uint32_t>(), sv->base->chars.as<uint8_t>(),
-	                                                  sv->base->offs.as<uint32_t>(), d, sv->cfg.measure,
-	                                                  w->flags.as<uint32_t>(), w->dist.as<int8_t>());
+	auto verify = sv->cfg.measure == 0 && d > 8 ? k_verify_cross<true> : k_verify_cross<false>;
+	verify<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->chars.as<uint8_t>(),
+	                                          w->offs.as<uint32_t>(), sv->base->chars.as<uint8_t>(),
+	                                          sv->base->offs.as<uint32_t>(), d, sv->cfg.measure,
+	                                          w->flags.as<uint32_t>(), w->dist.as<int8_t>());
 	HIP_LAUNCH_CHECK(w);
 	TRY(scan_excl<uint32_t>(w, w->flags.as<uint32_t>(), w->pos.as<uint32_t>(), np, w->pos.as<uint32_t>() + np));
 	uint32_t nout = 0;
@@ -1616,8 +1629,9 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 		hipStream_t s = h->stream;
 		TRY(ensure(h, h->pk, sizeof(uint64_t) * 2 * (size_t)n));
 		TRY(ensure(h, h->voff, sizeof(uint64_t) * ((size_t)n + 1)));
-		k_pack<<<grid_for(n, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), nullptr, 0, n,
-		                                         cfg.distance, h->pk.as<uint64_t>(), h->voff.as<uint64_t>());
+		k_pack<<<grid_for(n, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), nullptr, 0,
+		                                         nullptr, n, cfg.distance, h->pk.as<uint64_t>(),
+		                                         h->voff.as<uint64_t>());
 		HIP_LAUNCH_CHECK(h);
 		TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), n, h->voff.as<uint64_t>() + n));
 		uint64_t V = 0;

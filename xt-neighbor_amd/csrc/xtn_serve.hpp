@@ -73,6 +73,7 @@ __global__ __launch_bounds__(256) void k_match(const uint64_t* __restrict__ qrec
 
 // one lane per sorted candidate (query row, index row): duplicates skipped,
 // the rest verified between the query string and the index string
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_verify_cross(const uint64_t* __restrict__ pairs, int64_t n,
                                                       int idb, const uint8_t* __restrict__ qchars,
                                                       const uint32_t* __restrict__ qoffs,
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(256) void k_verify_cross(const uint64_t* __restrict
 	if (q == 0 || pairs[q - 1] != p) {
 		const uint32_t a = (uint32_t)(p >> idb), b = (uint32_t)(p & ((1ull << idb) - 1ull));
 		const uint32_t sa = qoffs[a], sb = ioffs[b];
-		const int d = pair_distance(qchars + sa, (int)(qoffs[a + 1] - sa), ichars + sb,
+		const int d = pair_distance<WIDE>(qchars + sa, (int)(qoffs[a + 1] - sa), ichars + sb,
 		                            (int)(ioffs[b + 1] - sb), distance, measure);
 		keep = d <= distance ? 1u : 0u;
 		dv = (int8_t)d;
