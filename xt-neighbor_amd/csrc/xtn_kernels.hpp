@@ -396,9 +396,27 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	const int v0 = s_v0;
 
 	const uint64_t idmask = (1ull << a.idb) - 1ull;
-	int idx = 0;
 	uint64_t kept[MODE == GEN_APPEND ? GEN_IT : 1];
 	uint32_t nkept = 0;
+	// The lane's first record: the sequence holding it (largest idx with
+	// s_vrel[idx] <= t) and its packed prefix. Every iteration then fetches
+	// the next record's sequence and prefix before generating its own, so the
+	// prefix load is in flight during the unranking and hashing.
+	int idx_n = 0;
+	uint64_t lo_n = 0, hi_n = 0;
+	if (r0 + t < r1) {
+		int lo = 0, hi = cnt;
+		while (lo < hi) {
+			int mid = (lo + hi + 1) >> 1;
+			if (s_vrel[mid] <= t)
+				lo = mid;
+			else
+				hi = mid - 1;
+		}
+		idx_n = lo;
+		lo_n = a.pk[2 * (slo + lo)];
+		hi_n = a.pk[2 * (slo + lo) + 1];
+	}
 	// kept rolled: the body holds the whole unranking, sixteen copies of it
 	// would not fit the instruction cache
 #pragma unroll 1
@@ -406,27 +424,21 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		const int rr = it * GEN_NT + t;  // record index within the tile
 		const int64_t r = r0 + rr;
 		const bool valid = r < r1;
+		const int idx = idx_n;
+		const uint64_t lo64 = lo_n, hiw = hi_n;
+		if (r + GEN_NT < r1) {  // records of a lane only move forward
+			const int rn = rr + GEN_NT;
+			int j = idx;
+			while (j + 1 < cnt && s_vrel[j + 1] <= rn)
+				++j;
+			idx_n = j;
+			lo_n = a.pk[2 * (slo + j)];
+			hi_n = a.pk[2 * (slo + j) + 1];
+		}
 		uint64_t rec = 0;
 		if (valid) {
-			// records of a lane only move forward: advance the sequence cursor
-			if (it == 0) {
-				int lo = 0, hi = cnt;  // largest idx with s_vrel[idx] <= rr
-				while (lo < hi) {
-					int mid = (lo + hi + 1) >> 1;
-					if (s_vrel[mid] <= rr)
-						lo = mid;
-					else
-						hi = mid - 1;
-				}
-				idx = lo;
-			} else {
-				while (idx + 1 < cnt && s_vrel[idx + 1] <= rr)
-					++idx;
-			}
 			const int64_t sid = slo + idx;
 			const uint32_t k = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_vrel[idx]));
-			const uint64_t lo64 = a.pk[2 * sid];
-			const uint64_t hiw = a.pk[2 * sid + 1];
 			const u128 p = ((u128)(hiw & PK_HI_MASK) << 64) | (u128)lo64;
 			u128 v;
 			if (small_d) {  // uniform: branch-free unranking
