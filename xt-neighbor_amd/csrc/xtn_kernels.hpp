@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "xtn_prims.hpp"
 
@@ -103,21 +104,72 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
 
 // ---------------------------------------------------------- generation --
 
-// 90-bit packed variant -> 64-bit key. The high bits enter through an odd
-// multiplier (injective), then one xorshift-multiply-xorshift round spreads
-// every input bit into the high bits the records are grouped by; one 64x64
-// multiply after the fold, half the quarter-rate multiplies of a two-round
-// finaliser. With
-// `wide` the high word also carries the query set (bits 26..55).
-template <bool wide>
-__device__ __forceinline__ uint64_t hash_variant(u128 v) {
-	const uint64_t lo = (uint64_t)v;
-	const uint64_t hi = wide ? (uint64_t)(v >> 64) : (uint64_t)(uint32_t)(v >> 64);  // 26 bits unless wide
-	uint64_t x = lo ^ (hi * 0x9E3779B97F4A7C15ull) ^ 0x5851F42D4C957F2Dull;
+// 90-bit packed variant -> 64-bit key, in two steps:
+//   H(v) = sum over the 18 slots i of slot_i(v) * G^i  (mod 2^64, G odd)
+//          + set * HASH_SET                              (batched form)
+//   key  = xorshift-multiply-xorshift of H
+// The polynomial makes deletions cheap: deleting slot q moves every later
+// slot down one place, which multiplies its term by G^-1. With
+// Pre[j] = sum_{i<j} slot_i(p) G^i, T = Pre[18] and
+// A[q] = Pre[q] - G^-1 Pre[q+1] (A[18] = T - G^-1 T), the variant of p with
+// slots q1 < q2 < ... < qm deleted (q = 18 stands for "no deletion") has
+//   H = A[q1] + G^-1 A[q2] + ... + G^-(m-1) A[qm] + G^-m T,
+// so a generator holding a sequence's A table and G^-m T pays two table
+// reads and one multiply per variant at m = 2. The key is a function of the
+// variant string alone (and the set), however it was reached.
+constexpr uint64_t HASH_G = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t inv_mod64(uint64_t a) {  // a odd; Newton's iteration, 3 -> 96 correct bits
+	uint64_t x = a;
+	for (int i = 0; i < 5; ++i)
+		x *= 2ull - a * x;
+	return x;
+}
+constexpr uint64_t HASH_GINV = inv_mod64(HASH_G);
+static_assert(HASH_G * HASH_GINV == 1ull, "G^-1");
+constexpr uint64_t HASH_GINV2 = HASH_GINV * HASH_GINV;
+constexpr uint64_t HASH_SET = 0x5851F42D4C957F2Dull;
+constexpr uint32_t PK_SLOT_HI_MASK = (1u << 26) - 1u;  // residue bits of the packed high word
+
+__device__ __forceinline__ uint64_t hash_mix(uint64_t x) {
 	x ^= x >> 32;
 	x *= 0xD6E8FEB86659FD93ull;
 	x ^= x >> 29;
 	return x;
+}
+
+__device__ __forceinline__ uint32_t slot_of(uint64_t lo, uint32_t hi26, int i) {
+	const int b = 5 * (MAX_PREFIX - 1 - i);  // slot 17 at bits 0..4, slot 0 at 85..89
+	const uint32_t x = b >= 64 ? (hi26 >> (b - 64)) : b > 59 ? (uint32_t)(lo >> b) | (hi26 << (64 - b))
+	                                                          : (uint32_t)(lo >> b);
+	return x & 31u;
+}
+
+// H of a packed variant (set id in bits 90..119 when `wide`), slot by slot:
+// the form for variants built explicitly
+constexpr uint64_t pow_mod64(uint64_t b, int e) {
+	uint64_t x = 1;
+	for (int i = 0; i < e; ++i)
+		x *= b;
+	return x;
+}
+constexpr uint64_t HASH_G17 = pow_mod64(HASH_G, MAX_PREFIX - 1);
+
+template <bool wide>
+__device__ __forceinline__ uint64_t hash_variant(u128 v) {
+	uint64_t lo = (uint64_t)v;
+	const uint64_t hi = (uint64_t)(v >> 64);
+	uint32_t hi26 = (uint32_t)hi & PK_SLOT_HI_MASK;
+	uint64_t h = 0, g = HASH_G17;
+#pragma unroll 1
+	for (int i = MAX_PREFIX - 1; i >= 0; --i) {  // slot i sits in the low 5 bits
+		h += (lo & 31u) * g;
+		lo = (lo >> 5) | ((uint64_t)hi26 << 59);
+		hi26 >>= 5;
+		g *= HASH_GINV;
+	}
+	if (wide)
+		h += ((hi >> PK_SET_SHIFT) & PK_SET_MASK) * HASH_SET;
+	return hash_mix(h);
 }
 
 // Deletes slot pos of a packed prefix (slot pos occupies bits
@@ -191,8 +243,11 @@ __constant__ GenTables c_gen_tables = make_gen_tables();
 // position: the sets inside a prefix of length L (largest position < L) are
 // then exactly the first sum_{j <= d} C(L, j) entries, so a variant's rank
 // indexes the table directly whatever L is. An entry holds three positions,
-// largest first, one byte each, NO_DEL where the set has fewer; applying all
-// three in that order deletes the set.
+// ascending in bytes 0, 1, 2, padded with NO_DEL (18) above the real ones
+// where the set has fewer: bytes 0..m-1 are the q1 < ... < qm of the
+// polynomial hash, and slot deletions applied from byte m-1 down to byte 0
+// (largest first, so the smaller positions still point at their slots)
+// build the variant.
 constexpr int DEL_SETS1 = 1 + 18;
 constexpr int DEL_SETS2 = DEL_SETS1 + 153;
 constexpr int DEL_SETS3 = DEL_SETS2 + 816;
@@ -216,10 +271,10 @@ constexpr DelTables make_del_tables() {
 		t.d2[q2++] = del_entry(m, NO_DEL, NO_DEL);
 		t.d3[q3++] = del_entry(m, NO_DEL, NO_DEL);
 		for (int j = 0; j < m; ++j) {
-			t.d2[q2++] = del_entry(m, j, NO_DEL);
-			t.d3[q3++] = del_entry(m, j, NO_DEL);
+			t.d2[q2++] = del_entry(j, m, NO_DEL);
+			t.d3[q3++] = del_entry(j, m, NO_DEL);
 			for (int i = 0; i < j; ++i)
-				t.d3[q3++] = del_entry(m, j, i);
+				t.d3[q3++] = del_entry(i, j, m);
 		}
 	}
 	return t;
@@ -330,11 +385,46 @@ __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n,
 // indexes, so stores are contiguous whatever the sequence lengths
 // (generate_combination.cu:212-229 instead runs one thread per sequence and
 // writes a strided run per lane).
+constexpr int GEN_TBL = 56;               // sequences per tile that get hash tables
+constexpr int GEN_TBL_W = MAX_PREFIX + 2;  // A[0..18], then G^-d T (+ the set term)
+
+// the hash table of one sequence (the A[q] and G^-d T of hash_variant's
+// comment), from its packed prefix
+__device__ __forceinline__ void seq_hash_table(const GenArgs& a, int64_t sid, uint64_t* __restrict__ tab) {
+	const uint64_t lo = a.pk[2 * sid];
+	const uint64_t hw = a.pk[2 * sid + 1];
+	const uint32_t hi26 = (uint32_t)hw & PK_SLOT_HI_MASK;
+	uint64_t pre = 0, g = 1;
+#pragma unroll
+	for (int q = 0; q < MAX_PREFIX; ++q) {
+		const uint64_t nxt = pre + (uint64_t)slot_of(lo, hi26, q) * g;
+		tab[q] = pre - HASH_GINV * nxt;
+		pre = nxt;
+		g *= HASH_G;
+	}
+	tab[MAX_PREFIX] = pre - HASH_GINV * pre;
+	uint64_t c = pre;
+	for (int j = 0; j < a.distance; ++j)
+		c *= HASH_GINV;
+	if (a.set_of)
+		c += ((hw >> PK_SET_SHIFT) & PK_SET_MASK) * HASH_SET;
+	tab[MAX_PREFIX + 1] = c;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	// variant offsets of the tile's sequences minus r0, clamped to
-	// [0, GEN_TILE]; the first one (<= 0) is kept whole in s_v0
-	__shared__ uint16_t s_vrel[GEN_TILE + 1];
+	// [0, GEN_TILE]; the first one (<= 0) is kept whole in s_v0. A tile of
+	// at most GEN_TBL sequences uses only the head of the offsets and keeps
+	// the sequences' hash tables behind it.
+	__shared__ union {
+		uint16_t vrel[GEN_TILE + 1];
+		struct {
+			uint16_t head[GEN_TBL + 4];
+			uint64_t tab[GEN_TBL][GEN_TBL_W];
+		} t;
+	} s_v;
+	uint16_t* const s_vrel = s_v.vrel;
 	__shared__ int32_t s_v0;
 	__shared__ union {
 		GenTables tab;              // d >= 4
@@ -392,6 +482,12 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		if (i == 0)
 			s_v0 = (int32_t)rel;
 	}
+	// uniform: hash tables for the tile's sequences, or variants built slot
+	// by slot (d >= 4, or more sequences than tables)
+	const bool tbl = small_d && cnt <= GEN_TBL;
+	if (tbl)
+		for (int i = t; i < cnt; i += GEN_NT)
+			seq_hash_table(a, slo + i, s_v.t.tab[i]);
 	__syncthreads();
 	const int v0 = s_v0;
 
@@ -399,79 +495,107 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	uint64_t kept[MODE == GEN_APPEND ? GEN_IT : 1];
 	uint32_t nkept = 0;
 	// The lane's first record: the sequence holding it (largest idx with
-	// s_vrel[idx] <= t) and its packed prefix. Every iteration then fetches
-	// the next record's sequence and prefix before generating its own, so the
-	// prefix load is in flight during the unranking and hashing.
-	int idx_n = 0;
-	uint64_t lo_n = 0, hi_n = 0;
-	if (r0 + t < r1) {
-		int lo = 0, hi = cnt;
-		while (lo < hi) {
-			int mid = (lo + hi + 1) >> 1;
-			if (s_vrel[mid] <= t)
-				lo = mid;
-			else
-				hi = mid - 1;
+	// s_vrel[idx] <= t). Every iteration then finds the next record's
+	// sequence before generating its own. The loop comes in two forms: with
+	// hash tables it reads nothing from global memory, so nothing waits on
+	// its stores; without, it also fetches the next record's packed prefix
+	// ahead, so that load is in flight during the unranking.
+	auto gen_loop = [&](auto tbl_c) {
+		constexpr bool TBL = decltype(tbl_c)::value;
+		int idx_n = 0;
+		uint64_t lo_n = 0, hi_n = 0;
+		if (r0 + t < r1) {
+			int lo = 0, hi = cnt;
+			while (lo < hi) {
+				int mid = (lo + hi + 1) >> 1;
+				if (s_vrel[mid] <= t)
+					lo = mid;
+				else
+					hi = mid - 1;
+			}
+			idx_n = lo;
+			if constexpr (!TBL) {
+				lo_n = a.pk[2 * (slo + lo)];
+				hi_n = a.pk[2 * (slo + lo) + 1];
+			}
 		}
-		idx_n = lo;
-		lo_n = a.pk[2 * (slo + lo)];
-		hi_n = a.pk[2 * (slo + lo) + 1];
-	}
-	// kept rolled: the body holds the whole unranking, sixteen copies of it
-	// would not fit the instruction cache
+		// kept rolled: the body holds the whole unranking, sixteen copies of
+		// it would not fit the instruction cache
 #pragma unroll 1
-	for (int it = 0; it < GEN_IT; ++it) {
-		const int rr = it * GEN_NT + t;  // record index within the tile
-		const int64_t r = r0 + rr;
-		const bool valid = r < r1;
-		const int idx = idx_n;
-		const uint64_t lo64 = lo_n, hiw = hi_n;
-		if (r + GEN_NT < r1) {  // records of a lane only move forward
-			const int rn = rr + GEN_NT;
-			int j = idx;
-			while (j + 1 < cnt && s_vrel[j + 1] <= rn)
-				++j;
-			idx_n = j;
-			lo_n = a.pk[2 * (slo + j)];
-			hi_n = a.pk[2 * (slo + j) + 1];
-		}
-		uint64_t rec = 0;
-		if (valid) {
-			const int64_t sid = slo + idx;
-			const uint32_t k = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_vrel[idx]));
-			const u128 p = ((u128)(hiw & PK_HI_MASK) << 64) | (u128)lo64;
-			u128 v;
-			if (small_d) {  // uniform: branch-free unranking
-				const uint32_t e = s_u.del[k];
-				v = del_slot_t(p, (int)(e & 0xFFu), s_hm, s_lm);
-				if (a.distance >= 2)
-					v = del_slot_t(v, (int)((e >> 8) & 0xFFu), s_hm, s_lm);
-				if (a.distance >= 3)
-					v = del_slot_t(v, (int)(e >> 16), s_hm, s_lm);
-			} else {
-				const int L = (int)(hiw >> 56);
-				const int m = a.distance < L ? a.distance : L;
-				v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
+		for (int it = 0; it < GEN_IT; ++it) {
+			const int rr = it * GEN_NT + t;  // record index within the tile
+			const int64_t r = r0 + rr;
+			const bool valid = r < r1;
+			const int idx = idx_n;
+			const uint64_t lo64 = lo_n, hiw = hi_n;
+			if (r + GEN_NT < r1) {  // records of a lane only move forward
+				const int rn = rr + GEN_NT;
+				int j = idx;
+				while (j + 1 < cnt && s_vrel[j + 1] <= rn)
+					++j;
+				idx_n = j;
+				if constexpr (!TBL) {
+					lo_n = a.pk[2 * (slo + j)];
+					hi_n = a.pk[2 * (slo + j) + 1];
+				}
 			}
-			// the set bits make variants of different query sets different keys
-			const uint64_t hv = a.set_of ? hash_variant<true>(v) : hash_variant<false>(v);
-			rec = (hv & ~idmask) | ((uint64_t)sid & idmask);
-		}
-		if (MODE == GEN_DIRECT) {
+			uint64_t rec = 0;
 			if (valid) {
-				a.out[r] = rec;
-				if (a.digit_hist)
-					atomicAdd(&s_dig[(uint32_t)(rec >> a.digit_shift) & 0xFFu], 1u);
+				const int64_t sid = slo + idx;
+				const uint32_t k = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_vrel[idx]));
+				uint64_t hv;
+				if constexpr (TBL) {  // the deletion set's table terms
+					const uint32_t e = s_u.del[k];
+					const uint64_t* tb = s_v.t.tab[idx];
+					uint64_t h = tb[MAX_PREFIX + 1];
+					if (a.distance >= 1)
+						h += tb[e & 0xFFu];
+					if (a.distance >= 2)
+						h += HASH_GINV * tb[(e >> 8) & 0xFFu];
+					if (a.distance >= 3)
+						h += HASH_GINV2 * tb[e >> 16];
+					hv = hash_mix(h);
+				} else {
+					const u128 p = ((u128)(hiw & PK_HI_MASK) << 64) | (u128)lo64;
+					u128 v;
+					if (small_d) {  // uniform: branch-free unranking, largest position first
+						const uint32_t e = s_u.del[k];
+						v = p;
+						if (a.distance >= 3)
+							v = del_slot_t(v, (int)(e >> 16), s_hm, s_lm);
+						if (a.distance >= 2)
+							v = del_slot_t(v, (int)((e >> 8) & 0xFFu), s_hm, s_lm);
+						v = del_slot_t(v, (int)(e & 0xFFu), s_hm, s_lm);
+					} else {
+						const int L = (int)(hiw >> 56);
+						const int m = a.distance < L ? a.distance : L;
+						v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
+					}
+					// the set bits make variants of different query sets different keys
+					hv = a.set_of ? hash_variant<true>(v) : hash_variant<false>(v);
+				}
+				rec = (hv & ~idmask) | ((uint64_t)sid & idmask);
 			}
-		} else if (MODE == GEN_HIST) {
-			if (valid)
-				atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1u);
-		} else {
-			const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
-			if (mine)
-				kept[nkept++ & (GEN_IT - 1)] = rec;
+			if (MODE == GEN_DIRECT) {
+				if (valid) {
+					a.out[r] = rec;
+					if (a.digit_hist)
+						atomicAdd(&s_dig[(uint32_t)(rec >> a.digit_shift) & 0xFFu], 1u);
+				}
+			} else if (MODE == GEN_HIST) {
+				if (valid)
+					atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1u);
+			} else {
+				const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
+				if (mine)
+					kept[nkept++ & (GEN_IT - 1)] = rec;
+			}
 		}
-	}
+	};
+	if (tbl)
+		gen_loop(std::true_type{});
+	else
+		gen_loop(std::false_type{});
 	if (MODE == GEN_APPEND) {
 		__shared__ uint32_t s_tmp[GEN_NT / WAVE + 1];
 		__shared__ unsigned long long s_base;
@@ -887,9 +1011,13 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 				const int j = i * BK_NT + t;
 				if (j >= cnt)
 					continue;
-				for (uint16_t n = s_next[j]; n != BK_NONE; n = s_next[n]) {
+				for (uint32_t n = s_next[j]; n != BK_NONE;) {
+					const uint64_t kn = s_keys[n];  // key and link read together
+					const uint32_t nn = s_next[n];
 					uint64_t p;
-					if (!pair_of(k[i], s_keys[n], a.idb, &p))
+					const bool ok = pair_of(k[i], kn, a.idb, &p);
+					n = nn;
+					if (!ok)
 						continue;
 					const uint32_t slot = atomicAdd(&s_nstage, 1u);
 					if (slot < (uint32_t)BK_STAGE) {
