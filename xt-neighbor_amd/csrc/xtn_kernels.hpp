@@ -502,7 +502,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	// ahead, so that load is in flight during the unranking.
 	auto gen_loop = [&](auto tbl_c) {
 		constexpr bool TBL = decltype(tbl_c)::value;
-		int idx_n = 0;
+		int idx_n = 0, start_n = 0;  // start: the sequence's first record in the tile (v0 for idx 0)
 		uint64_t lo_n = 0, hi_n = 0;
 		if (r0 + t < r1) {
 			int lo = 0, hi = cnt;
@@ -514,6 +514,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 					hi = mid - 1;
 			}
 			idx_n = lo;
+			start_n = lo == 0 ? v0 : (int)s_vrel[lo];
 			if constexpr (!TBL) {
 				lo_n = a.pk[2 * (slo + lo)];
 				hi_n = a.pk[2 * (slo + lo) + 1];
@@ -526,14 +527,33 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			const int rr = it * GEN_NT + t;  // record index within the tile
 			const int64_t r = r0 + rr;
 			const bool valid = r < r1;
-			const int idx = idx_n;
+			const int idx = idx_n, start = start_n;
 			const uint64_t lo64 = lo_n, hiw = hi_n;
 			if (r + GEN_NT < r1) {  // records of a lane only move forward
+				// the next four sequence starts per look (the offsets do not
+				// decrease; indexes past cnt count as "beyond")
 				const int rn = rr + GEN_NT;
-				int j = idx;
-				while (j + 1 < cnt && s_vrel[j + 1] <= rn)
-					++j;
+				int j = idx, st = start;
+				for (;;) {
+					int sv[4];
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						const int q = j + 1 + u;
+						sv[u] = q < cnt ? (int)s_vrel[q < GEN_TILE ? q : GEN_TILE] : GEN_TILE + 1;
+					}
+					int nadv = 0;
+#pragma unroll
+					for (int u = 0; u < 4; ++u)
+						if (sv[u] <= rn) {
+							nadv = u + 1;
+							st = sv[u];
+						}
+					j += nadv;
+					if (nadv < 4)
+						break;
+				}
 				idx_n = j;
+				start_n = st;
 				if constexpr (!TBL) {
 					lo_n = a.pk[2 * (slo + j)];
 					hi_n = a.pk[2 * (slo + j) + 1];
@@ -542,18 +562,20 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			uint64_t rec = 0;
 			if (valid) {
 				const int64_t sid = slo + idx;
-				const uint32_t k = (uint32_t)(rr - (idx == 0 ? v0 : (int)s_vrel[idx]));
+				const uint32_t k = (uint32_t)(rr - start);
 				uint64_t hv;
 				if constexpr (TBL) {  // the deletion set's table terms
 					const uint32_t e = s_u.del[k];
 					const uint64_t* tb = s_v.t.tab[idx];
+					// every byte of an entry is <= NO_DEL: the three reads go out together
+					const uint64_t t1 = tb[e & 0xFFu], t2 = tb[(e >> 8) & 0xFFu], t3 = tb[e >> 16];
 					uint64_t h = tb[MAX_PREFIX + 1];
 					if (a.distance >= 1)
-						h += tb[e & 0xFFu];
+						h += t1;
 					if (a.distance >= 2)
-						h += HASH_GINV * tb[(e >> 8) & 0xFFu];
+						h += HASH_GINV * t2;
 					if (a.distance >= 3)
-						h += HASH_GINV2 * tb[e >> 16];
+						h += HASH_GINV2 * t3;
 					hv = hash_mix(h);
 				} else {
 					const u128 p = ((u128)(hiw & PK_HI_MASK) << 64) | (u128)lo64;
