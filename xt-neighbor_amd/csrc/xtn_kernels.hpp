@@ -363,6 +363,7 @@ struct GenArgs {
 	unsigned long long* part_hist;  // GEN_HIST: 2^part_bits counters
 	unsigned long long* append;     // GEN_APPEND cursor
 	const uint32_t* set_of;         // non-null in the batched form: the set ids k_pack put in pk
+	const uint32_t* tile_seq;       // nullable: k_tile_seq's table (else a binary search)
 	uint32_t* digit_hist;           // GEN_DIRECT, nullable: per-tile counts of the
 	int64_t ntiles;                 // 8-bit digit at digit_shift, one row of
 	int32_t digit_shift;            // RS_BINS per tile as k_radix_hist writes them
@@ -379,6 +380,21 @@ __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n,
 			hi = mid;
 	}
 	return lo;
+}
+
+// tile_seq[b] = the sequence holding record b * GEN_TILE for the ntiles
+// tiles of V records, and tile_seq[ntiles] = the one holding record V - 1:
+// each sequence writes the entries of the tile starts inside its records
+__global__ __launch_bounds__(256) void k_tile_seq(const uint64_t* __restrict__ voff, int32_t n, uint64_t V,
+                                                  uint32_t ntiles, uint32_t* __restrict__ tile_seq) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n)
+		return;
+	const uint64_t o = voff[i], e = voff[i + 1];
+	for (uint64_t b = (o + GEN_TILE - 1) / GEN_TILE; b * GEN_TILE < e; ++b)
+		tile_seq[b] = (uint32_t)i;
+	if (o < e && o <= V - 1 && V - 1 < e)
+		tile_seq[ntiles] = (uint32_t)i;
 }
 
 // Flattened generation: each workgroup owns GEN_TILE consecutive record
@@ -462,9 +478,18 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	}
 	if (t == 0) {
 		// voff[n] == V, so both searches land in [0, n-1]; the clamps only
-		// keep a caller's inconsistent V from reading past voff
-		int64_t lo = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)r0) - 1;
-		int64_t hi = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)(r1 - 1)) - 1;
+		// keep a caller's inconsistent V from reading past voff. The table
+		// gives the sequence of the next tile's first record as hi, one
+		// sequence more than needed where that sequence starts the next
+		// tile: its offset clamps to GEN_TILE and no record selects it.
+		int64_t lo, hi;
+		if (a.tile_seq) {
+			lo = a.tile_seq[blockIdx.x];
+			hi = a.tile_seq[blockIdx.x + 1];
+		} else {
+			lo = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)r0) - 1;
+			hi = upper_bound_u64(a.voff, (int64_t)a.n + 1, (uint64_t)(r1 - 1)) - 1;
+		}
 		lo = lo < a.n - 1 ? lo : a.n - 1;
 		hi = hi < a.n - 1 ? hi : a.n - 1;
 		hi = hi > lo ? hi : lo;

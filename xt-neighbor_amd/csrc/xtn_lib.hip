@@ -87,7 +87,7 @@ struct xtn_handle {
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAcc> prof;
 	std::vector<PendingEvent> spans;  // stage timing, always on
-	DBuf bstart, ovf, set_of, set_offs;
+	DBuf bstart, ovf, set_of, set_offs, tseq;
 	int64_t last_pairs = 0;
 	// host copy of the prefix lengths, and per-row prefix sums of the
 	// variant counts for one distance: V of any row range without a readback
@@ -198,6 +198,18 @@ uint64_t host_variant_count(uint32_t len, int d) {
 inline unsigned grid_for(int64_t n, int per_block) {
 	int64_t g = (n + per_block - 1) / per_block;
 	return (unsigned)(g > 0 ? g : 1);
+}
+
+// the generator's table of each tile's first sequence, over the voff of n
+// sequences holding V >= 1 records
+static int tile_seq_table(xtn_handle* h, hipStream_t s, int32_t n, uint64_t V, const uint32_t** out) {
+	const uint64_t ntiles = (V + GEN_TILE - 1) / GEN_TILE;
+	TRY(ensure(h, h->tseq, sizeof(uint32_t) * (size_t)(ntiles + 1)));
+	k_tile_seq<<<grid_for(n, 256), 256, 0, s>>>(h->voff.as<uint64_t>(), n, V, (uint32_t)ntiles,
+	                                            h->tseq.as<uint32_t>());
+	HIP_LAUNCH_CHECK(h);
+	*out = h->tseq.as<uint32_t>();
+	return XTN_OK;
 }
 
 // ------------------------------------------------------------ profiling --
@@ -589,6 +601,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	ga.part_bits = pbits;
 	ga.V = (int64_t)V;
 	ga.set_of = set_of;
+	if (V > 0)
+		TRY(tile_seq_table(h, s, nseq, V, &ga.tile_seq));
 	const unsigned gen_grid = grid_for((int64_t)V, GEN_TILE);
 
 	std::vector<unsigned long long> part_count(nparts, 0);
@@ -1090,7 +1104,7 @@ void xtn_close(xtn_handle* h) {
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
 	                &h->ov_vals_alt, &h->ov_info, &h->ov_leader, &h->ov_ustart, &h->ov_umap,
 	                &h->ov_cnt, &h->ov_matrix, &h->ov_present, &h->bstart, &h->ovf, &h->set_of,
-	                &h->set_offs};
+	                &h->set_offs, &h->tseq};
 	for (DBuf* b : bufs)
 		release(*b);
 	for (auto& pe : h->pending) {
@@ -1484,6 +1498,8 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	ga.V = V;
 	ga.out = w->recs.as<uint64_t>();
 	ga.cap = V;
+	if (V > 0)
+		TRY(tile_seq_table(w, s, n, (uint64_t)V, &ga.tile_seq));
 	k_gen<GEN_DIRECT><<<grid_for(V, GEN_TILE), GEN_NT, 0, s>>>(ga);
 	HIP_LAUNCH_CHECK(w);
 	int64_t cap = std::max<int64_t>({w->last_pairs + w->last_pairs / 4, V / 2, 1 << 16});
@@ -1650,6 +1666,8 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 		ga.V = (int64_t)V;
 		ga.out = h->recs.as<uint64_t>();
 		ga.cap = (int64_t)V;
+		if (V > 0)
+			TRY(tile_seq_table(h, s, n, V, &ga.tile_seq));
 		k_gen<GEN_DIRECT><<<grid_for((int64_t)V, GEN_TILE), GEN_NT, 0, s>>>(ga);
 		HIP_LAUNCH_CHECK(h);
 		TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, (int64_t)V, sv->idb, 64, "index_sort"));
