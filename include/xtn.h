@@ -84,8 +84,8 @@ typedef struct {
 	int32_t profile;               /* 1: time every kernel launch with HIP
 	                                  events (see xtn_kernel_profile) */
 	int32_t bucket_capacity;       /* records one workgroup groups in LDS
-	                                  (0: 4096); larger buckets take the
-	                                  global sort path */
+	                                  (0 or above 3072: 3072); larger
+	                                  buckets take the global sort path */
 } xtn_config;
 
 typedef struct xtn_handle xtn_handle;

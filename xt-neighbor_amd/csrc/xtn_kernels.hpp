@@ -945,7 +945,7 @@ struct BucketArgs {
 	int32_t bin_shift;  // the bucket is binned by bits [bin_shift, bin_shift + bin_bits)
 	int32_t bin_bits;   // <= BK_BIN_BITS
 	int32_t idb;
-	int32_t cap;        // largest bucket handled in LDS (<= RS_TILE)
+	int32_t cap;        // largest bucket handled in LDS (<= BK_CAP)
 	int32_t ovf_cap;
 	uint64_t* pairs;
 	int64_t pcap;
@@ -965,16 +965,17 @@ __device__ __forceinline__ void bucket_range(const BucketArgs& a, int b, uint32_
 	}
 }
 
-constexpr int BK_BIN_BITS = 11;
+constexpr int BK_BIN_BITS = 12;
 constexpr int BK_BINS = 1 << BK_BIN_BITS;
 constexpr int BK_NT = 512;
-constexpr int BK_PER = RS_TILE / BK_NT;  // records per thread
+constexpr int BK_CAP = 3072;             // largest bucket grouped in LDS
+constexpr int BK_PER = BK_CAP / BK_NT;   // records per thread
 constexpr uint16_t BK_NONE = 0xFFFF;
 
 constexpr int BK_STAGE = 384;  // pairs staged in LDS between flushes
 
-// Buckets of <= RS_TILE records, each handled by one workgroup in LDS: every
-// record is pushed on the list of its bin (11 more hash bits, one atomicExch
+// Buckets of <= BK_CAP records, each handled by one workgroup in LDS: every
+// record is pushed on the list of its bin (12 more hash bits, one atomicExch
 // each), then each record walks the records pushed before it in its bin and
 // emits the pairs that share the full stored hash and hold two different
 // ids. Every unordered pair of a bin is visited exactly once; the emission
@@ -983,8 +984,8 @@ constexpr int BK_STAGE = 384;  // pairs staged in LDS between flushes
 // loading the records of its next bucket into registers while it works on
 // the current one, and collects pairs in LDS (LDS atomics), appending them to
 // the global buffer with one atomic per flush. Every loop is bounded by the
-// bucket count; nothing waits on another workgroup. 52 KB of LDS: three
-// workgroups (24 waves) per CU.
+// bucket count; nothing waits on another workgroup. 49 KB of LDS: three
+// workgroups (24 waves) per CU (at 53.8 KB only two fit on MI355X).
 // records of bucket b into registers (nothing for an overflowing bucket)
 __device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t (&k)[BK_PER],
                                             uint32_t* s, uint32_t* e) {
@@ -1001,8 +1002,8 @@ __device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t
 }
 
 __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
-	__shared__ uint64_t s_keys[RS_TILE];
-	__shared__ uint16_t s_next[RS_TILE];
+	__shared__ uint64_t s_keys[BK_CAP];
+	__shared__ uint16_t s_next[BK_CAP];
 	__shared__ uint32_t s_head[BK_BINS];  // (iteration << 16) | record of the bin's last push
 	__shared__ uint64_t s_stage[BK_STAGE];
 	__shared__ uint32_t s_nstage;
@@ -1043,14 +1044,23 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 			}
 		} else if (cnt >= 2) {
 			const uint32_t tag = iter << 16;
+			// each record's link also stays in a register, two per register,
+			// so the walk starts without reading it back
+			uint32_t pv[BK_PER / 2];
 #pragma unroll
 			for (int i = 0; i < BK_PER; ++i) {
 				const int j = i * BK_NT + t;
+				uint32_t link = BK_NONE;
 				if (j < cnt) {
 					s_keys[j] = k[i];
 					const uint32_t prev = atomicExch(&s_head[(uint32_t)(k[i] >> a.bin_shift) & bmask], tag | (uint32_t)j);
-					s_next[j] = (prev & 0xFFFF0000u) == tag ? (uint16_t)prev : BK_NONE;
+					link = (prev & 0xFFFF0000u) == tag ? (prev & 0xFFFFu) : BK_NONE;
+					s_next[j] = (uint16_t)link;
 				}
+				if (i & 1)
+					pv[i / 2] |= link << 16;
+				else
+					pv[i / 2] = link;
 			}
 			__syncthreads();
 #pragma unroll
@@ -1058,7 +1068,7 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 				const int j = i * BK_NT + t;
 				if (j >= cnt)
 					continue;
-				for (uint32_t n = s_next[j]; n != BK_NONE;) {
+				for (uint32_t n = (pv[i / 2] >> (16 * (i & 1))) & 0xFFFFu; n != BK_NONE;) {
 					const uint64_t kn = s_keys[n];  // key and link read together
 					const uint32_t nn = s_next[n];
 					uint64_t p;

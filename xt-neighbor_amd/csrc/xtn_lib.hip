@@ -434,7 +434,7 @@ double now_ms() {
 
 // ----------------------------------------------------------- NN pipeline --
 
-constexpr int64_t BUCKET_TARGET = 3500;  // mean records per LDS bucket (<= 4096 fit)
+constexpr int64_t BUCKET_TARGET = 2200;  // mean records per LDS bucket (<= BK_CAP fit)
 constexpr int KEY_BITS = 32;             // hash bits that order records into runs
 
 inline unsigned long long* ctr_field(xtn_handle* h, size_t offset) {
@@ -636,7 +636,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	TRY(ensure(h, h->bcount, sizeof(uint32_t) * 65536));
 	TRY(ensure(h, h->ctl, sizeof(uint32_t) * CTL_WORDS));
 	TRY(ensure(h, h->ovf, sizeof(uint32_t) * 65536));
-	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, RS_TILE) : RS_TILE;
+	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, BK_CAP) : BK_CAP;
 
 	// candidate pair buffer, sized from the last run; regrown on overflow
 	const int64_t pguess = std::max<int64_t>({h->last_pairs + h->last_pairs / 4, (int64_t)V / 16, 1 << 16});
