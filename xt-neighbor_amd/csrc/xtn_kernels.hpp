@@ -911,8 +911,8 @@ __global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
 		const bool valid = r * WAVE + lane < w_valid;
 		k[r] = valid ? a.in[seg + (int64_t)r * WAVE + lane] : 0ull;
 	}
-	uint32_t lp[IT];
-	rank_tile_g<NT, IT>(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);
+	uint32_t lp[IT];  // slot order inside a bucket is free: the unstable rank
+	rank_tile_u<NT, IT>(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);
 #pragma unroll
 	for (int r = 0; r < IT; ++r)
 		if (lp[r] != 0xFFFFFFFFu)

@@ -382,6 +382,46 @@ __device__ __forceinline__ void rank_tile_g(const uint64_t (&k)[IT], int w_valid
 		lp[r] = rk[r] == 0xFFFFFFFFu ? 0xFFFFFFFFu : s_cnt[w][rk[r] & 0xFFu] + (rk[r] >> 8);
 }
 
+// The same outputs without stability: one LDS atomic per key on a single
+// tile histogram (s_cnt[0]) gives its rank inside its digit, in whatever
+// order the atomics land. For passes whose output order inside a digit run
+// does not matter; about a tenth of rank_tile_g's VALU instructions.
+template <int NT, int IT>
+__device__ __forceinline__ void rank_tile_u(const uint64_t (&k)[IT], int w_valid, int shift,
+                                            uint32_t (&lp)[IT], uint32_t (*s_cnt)[RS_BINS],
+                                            uint32_t* s_dpre, uint32_t* s_scan) {
+	static_assert(NT % WAVE == 0 && NT >= RS_BINS && NT / WAVE >= 2,
+	              "one thread per digit in the prefix step; s_cnt[1][0] is the spare bin");
+	const int t = threadIdx.x;
+	const int lane = lane_id();
+	uint32_t* s_h = s_cnt[0];
+	for (int i = t; i <= RS_BINS; i += NT)
+		s_h[i] = 0;
+	__syncthreads();
+	// no branch around the atomics, so they are all in flight together:
+	// lanes without a key count into the spare bin s_h[RS_BINS]
+	uint32_t rk[IT];
+#pragma unroll
+	for (int r = 0; r < IT; ++r) {
+		const bool valid = r * WAVE + lane < w_valid;
+		const uint32_t d = valid ? (uint32_t)(k[r] >> shift) & 0xFFu : (uint32_t)RS_BINS;
+		rk[r] = atomicAdd(&s_h[d], 1u);
+		rk[r] = valid ? ((rk[r] << 8) | d) : 0xFFFFFFFFu;
+	}
+	__syncthreads();
+	{
+		const int d = t;
+		const uint32_t c = d < RS_BINS ? s_h[d] : 0u;
+		const uint32_t ex = block_excl_scan<NT>(c, s_scan, (uint32_t*)nullptr);
+		if (d < RS_BINS)
+			s_dpre[d] = ex;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < IT; ++r)
+		lp[r] = rk[r] == 0xFFFFFFFFu ? 0xFFFFFFFFu : s_dpre[rk[r] & 0xFFu] + (rk[r] >> 8);
+}
+
 template <int IT>
 __device__ __forceinline__ void rank_tile_n(const uint64_t (&k)[IT], int w_valid, int shift,
                                             uint32_t (&lp)[IT], uint32_t (*s_cnt)[RS_BINS],
@@ -395,7 +435,7 @@ __device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_vali
 	rank_tile_n<RS_IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 }
 
-template <bool KV, int NT = RS_NT>
+template <bool KV, int NT = RS_NT, bool STABLE = true>
 __global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
                                                          uint64_t* __restrict__ keys_out,
@@ -432,7 +472,10 @@ __global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict
 	if (t < RS_BINS)
 		s_gbase[t] = offs[tile * RS_BINS + t];
 	uint32_t lp[IT];
-	rank_tile_g<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+	if constexpr (STABLE)
+		rank_tile_g<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+	else
+		rank_tile_u<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 #pragma unroll
 	for (int r = 0; r < IT; ++r) {
 		if (lp[r] != 0xFFFFFFFFu) {
