@@ -89,6 +89,8 @@ struct xtn_handle {
 	std::vector<PendingEvent> spans;  // stage timing, always on
 	DBuf bstart, ovf, set_of, set_offs, tseq;
 	int64_t last_pairs = 0;
+	int64_t last_unique = 0, last_out = 0;  // the previous run's unique candidates and outputs
+	                                         // (byte counts of the output select's profile)
 	// host copy of the prefix lengths, and per-row prefix sums of the
 	// variant counts for one distance: V of any row range without a readback
 	int n_cu = 256;
@@ -849,24 +851,29 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(kcnt, nv, kcnt + nv);
 			HIP_LAUNCH_CHECK(h);
 		}
+		// the outputs hold every unique pair (at most n_pairs), so the select
+		// is launched before the totals come back: the device does not wait
+		// on the host between verification and output; workgroups past the
+		// unique count exit at once
+		TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)n_pairs));
+		TRY(ensure(h, h->out_dist, (size_t)n_pairs));
+		{
+			ProfScope ps(h, "emit", (double)h->last_unique * 9.0 + (double)h->last_out * 9.0);
+			k_keep_write<<<(unsigned)nv, SEL_NT, 0, s>>>(upairs, ucnt + nt, idb, h->dist.as<int8_t>(), kcnt,
+			                                             h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
+			HIP_LAUNCH_CHECK(h);
+		}
 		uint32_t totals[2] = {0, 0};
 		HIP_TRY(h, hipMemcpyAsync(&totals[0], ucnt + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		HIP_TRY(h, hipMemcpyAsync(&totals[1], kcnt + nv, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 		HIP_TRY(h, hipStreamSynchronize(s));
 		st.n_unique = (int64_t)totals[0];
+		h->last_unique = st.n_unique;
 		nout = totals[1];
-		TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
-		TRY(ensure(h, h->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
-		if (nout > 0) {
-			const unsigned grid = (unsigned)((totals[0] + SEL_NT - 1) / SEL_NT);
-			ProfScope ps(h, "emit", (double)totals[0] * 9.0 + (double)nout * 9.0);
-			k_keep_write<<<grid, SEL_NT, 0, s>>>(upairs, ucnt + nt, idb, h->dist.as<int8_t>(), kcnt,
-			                                     h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
-			HIP_LAUNCH_CHECK(h);
-		}
 	}
 	HIP_TRY(h, hipStreamSynchronize(s));
 	h->out_len = nout;
+	h->last_out = nout;
 	st.n_output = h->out_len;
 	st.ms_total = now_ms() - t0;
 	collect_stages(h);
