@@ -67,6 +67,28 @@ def test_large_groups_use_flattened_pairs(engine):
     assert np.array_equal(got, oracle.nn_brute(chars, offsets, 2, 0))
 
 
+@pytest.mark.parametrize("d,base_len,ins", [(2, 10, 2), (3, 6, 3)])
+def test_hash_paths_agree_across_tiles(engine, d, base_len, ins):
+    # at d = 2 and 3 the generator keys the variants of a tile of at most 56
+    # sequences from per-sequence tables and builds those of a fuller tile
+    # slot by slot: here the originals (length 10 at d = 2: ~73 sequences per
+    # tile; length 6 at d = 3: ~97) take the second path and their insertion
+    # relatives (length 12: ~52 per tile; length 9: ~31) the first, and every
+    # pair between the two halves must still share a key
+    rng = np.random.default_rng(40 + d)
+    base = random_seqs(rng, 3000, base_len, base_len)
+    rel = []
+    for s in base[:1500]:
+        t = list(s)
+        for _ in range(ins):
+            t.insert(int(rng.integers(0, len(t) + 1)), AA20[int(rng.integers(20))])
+        rel.append("".join(t))
+    chars, offsets = xtn.encode(base + rel)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=d)
+    assert np.array_equal(got, oracle.nn_port(chars, offsets, d, 0)[0])
+    assert sum(1 for i, j, _ in got.tolist() if i < 3000 <= j) >= 1500
+
+
 @pytest.mark.parametrize("copies", [3, 4, 7])
 def test_runs_across_tile_boundaries(engine, copies):
     # every string repeated: thousands of runs of 3..10 records, many of

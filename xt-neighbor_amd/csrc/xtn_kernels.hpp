@@ -144,8 +144,6 @@ __device__ __forceinline__ uint32_t slot_of(uint64_t lo, uint32_t hi26, int i) {
 	return x & 31u;
 }
 
-// H of a packed variant (set id in bits 90..119 when `wide`), slot by slot:
-// the form for variants built explicitly
 constexpr uint64_t pow_mod64(uint64_t b, int e) {
 	uint64_t x = 1;
 	for (int i = 0; i < e; ++i)
@@ -154,6 +152,25 @@ constexpr uint64_t pow_mod64(uint64_t b, int e) {
 }
 constexpr uint64_t HASH_G17 = pow_mod64(HASH_G, MAX_PREFIX - 1);
 
+// Distances 2 and 3 key records by H (their tiles hold few enough sequences
+// for the tables); other distances, whose variants are always built
+// explicitly, by hash_word: the variant's two words through an odd
+// multiplier and the same finaliser. Both are functions of the variant (and
+// the set) alone; one pipeline run, and the index and queries of the served
+// form, use one distance and therefore one of them.
+__device__ __forceinline__ bool poly_keys(int distance) {
+	return distance == 2 || distance == 3;
+}
+
+template <bool wide>
+__device__ __forceinline__ uint64_t hash_word(u128 v) {
+	const uint64_t lo = (uint64_t)v;
+	const uint64_t hi = wide ? (uint64_t)(v >> 64) : (uint64_t)(uint32_t)(v >> 64);  // 26 bits unless wide
+	return hash_mix(lo ^ (hi * HASH_G) ^ HASH_SET);
+}
+
+// H of a packed variant (set id in bits 90..119 when `wide`), slot by slot:
+// the form for variants built explicitly
 template <bool wide>
 __device__ __forceinline__ uint64_t hash_variant(u128 v) {
 	uint64_t lo = (uint64_t)v;
@@ -509,7 +526,8 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	}
 	// uniform: hash tables for the tile's sequences, or variants built slot
 	// by slot (d >= 4, or more sequences than tables)
-	const bool tbl = small_d && cnt <= GEN_TBL;
+	const bool poly = poly_keys(a.distance);
+	const bool tbl = poly && cnt <= GEN_TBL;
 	if (tbl)
 		for (int i = t; i < cnt; i += GEN_NT)
 			seq_hash_table(a, slo + i, s_v.t.tab[i]);
@@ -619,7 +637,10 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 						v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
 					}
 					// the set bits make variants of different query sets different keys
-					hv = a.set_of ? hash_variant<true>(v) : hash_variant<false>(v);
+					if (poly)
+						hv = a.set_of ? hash_variant<true>(v) : hash_variant<false>(v);
+					else
+						hv = a.set_of ? hash_word<true>(v) : hash_word<false>(v);
 				}
 				rec = (hv & ~idmask) | ((uint64_t)sid & idmask);
 			}
