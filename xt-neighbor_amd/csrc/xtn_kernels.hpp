@@ -419,6 +419,7 @@ __global__ __launch_bounds__(256) void k_tile_seq(const uint64_t* __restrict__ v
 // (generate_combination.cu:212-229 instead runs one thread per sequence and
 // writes a strided run per lane).
 constexpr int GEN_TBL = 56;               // sequences per tile that get hash tables
+constexpr int GEN_PKL = 448;              // sequences per tile whose prefixes are staged in LDS
 constexpr int GEN_TBL_W = MAX_PREFIX + 2;  // A[0..18], then G^-d T (+ the set term)
 
 // the hash table of one sequence (the A[q] and G^-d T of hash_variant's
@@ -449,13 +450,18 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	// variant offsets of the tile's sequences minus r0, clamped to
 	// [0, GEN_TILE]; the first one (<= 0) is kept whole in s_v0. A tile of
 	// at most GEN_TBL sequences uses only the head of the offsets and keeps
-	// the sequences' hash tables behind it.
+	// the sequences' hash tables behind it; one of at most GEN_PKL, their
+	// packed prefixes.
 	__shared__ union {
 		uint16_t vrel[GEN_TILE + 1];
 		struct {
 			uint16_t head[GEN_TBL + 4];
 			uint64_t tab[GEN_TBL][GEN_TBL_W];
 		} t;
+		struct {
+			uint16_t head[GEN_PKL + 8];
+			uint64_t pk[GEN_PKL][2];
+		} p;
 	} s_v;
 	uint16_t* const s_vrel = s_v.vrel;
 	__shared__ int32_t s_v0;
@@ -524,13 +530,19 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		if (i == 0)
 			s_v0 = (int32_t)rel;
 	}
-	// uniform: hash tables for the tile's sequences, or variants built slot
-	// by slot (d >= 4, or more sequences than tables)
+	// uniform: hash tables for the tile's sequences (form 0), or variants
+	// built slot by slot from prefixes staged in LDS (form 1) or read from
+	// global memory (form 2, tiles of more than GEN_PKL sequences)
 	const bool poly = poly_keys(a.distance);
-	const bool tbl = poly && cnt <= GEN_TBL;
-	if (tbl)
+	const int form = poly && cnt <= GEN_TBL ? 0 : cnt <= GEN_PKL ? 1 : 2;
+	if (form == 0)
 		for (int i = t; i < cnt; i += GEN_NT)
 			seq_hash_table(a, slo + i, s_v.t.tab[i]);
+	else if (form == 1)
+		for (int i = t; i < cnt; i += GEN_NT) {
+			s_v.p.pk[i][0] = a.pk[2 * (slo + i)];
+			s_v.p.pk[i][1] = a.pk[2 * (slo + i) + 1];
+		}
 	__syncthreads();
 	const int v0 = s_v0;
 
@@ -539,12 +551,13 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	uint32_t nkept = 0;
 	// The lane's first record: the sequence holding it (largest idx with
 	// s_vrel[idx] <= t). Every iteration then finds the next record's
-	// sequence before generating its own. The loop comes in two forms: with
-	// hash tables it reads nothing from global memory, so nothing waits on
-	// its stores; without, it also fetches the next record's packed prefix
-	// ahead, so that load is in flight during the unranking.
-	auto gen_loop = [&](auto tbl_c) {
-		constexpr bool TBL = decltype(tbl_c)::value;
+	// sequence before generating its own. Forms 0 and 1 read nothing from
+	// global memory inside the loop, so nothing waits on its stores; form 2
+	// fetches the next record's packed prefix ahead, so that load is in
+	// flight during the unranking.
+	auto gen_loop = [&](auto form_c) {
+		constexpr int FORM = decltype(form_c)::value;
+		constexpr bool TBL = FORM == 0;
 		int idx_n = 0, start_n = 0;  // start: the sequence's first record in the tile (v0 for idx 0)
 		uint64_t lo_n = 0, hi_n = 0;
 		if (r0 + t < r1) {
@@ -558,7 +571,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			}
 			idx_n = lo;
 			start_n = lo == 0 ? v0 : (int)s_vrel[lo];
-			if constexpr (!TBL) {
+			if constexpr (FORM == 2) {
 				lo_n = a.pk[2 * (slo + lo)];
 				hi_n = a.pk[2 * (slo + lo) + 1];
 			}
@@ -597,7 +610,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 				}
 				idx_n = j;
 				start_n = st;
-				if constexpr (!TBL) {
+				if constexpr (FORM == 2) {
 					lo_n = a.pk[2 * (slo + j)];
 					hi_n = a.pk[2 * (slo + j) + 1];
 				}
@@ -621,7 +634,12 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 						h += HASH_GINV2 * t3;
 					hv = hash_mix(h);
 				} else {
-					const u128 p = ((u128)(hiw & PK_HI_MASK) << 64) | (u128)lo64;
+					uint64_t plo = lo64, phi = hiw;
+					if constexpr (FORM == 1) {
+						plo = s_v.p.pk[idx][0];
+						phi = s_v.p.pk[idx][1];
+					}
+					const u128 p = ((u128)(phi & PK_HI_MASK) << 64) | (u128)plo;
 					u128 v;
 					if (small_d) {  // uniform: branch-free unranking, largest position first
 						const uint32_t e = s_u.del[k];
@@ -632,7 +650,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 							v = del_slot_t(v, (int)((e >> 8) & 0xFFu), s_hm, s_lm);
 						v = del_slot_t(v, (int)(e & 0xFFu), s_hm, s_lm);
 					} else {
-						const int L = (int)(hiw >> 56);
+						const int L = (int)(phi >> 56);
 						const int m = a.distance < L ? a.distance : L;
 						v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
 					}
@@ -660,10 +678,12 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			}
 		}
 	};
-	if (tbl)
-		gen_loop(std::true_type{});
+	if (form == 0)
+		gen_loop(std::integral_constant<int, 0>{});
+	else if (form == 1)
+		gen_loop(std::integral_constant<int, 1>{});
 	else
-		gen_loop(std::false_type{});
+		gen_loop(std::integral_constant<int, 2>{});
 	if (MODE == GEN_APPEND) {
 		__shared__ uint32_t s_tmp[GEN_NT / WAVE + 1];
 		__shared__ unsigned long long s_base;
