@@ -83,9 +83,26 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
 	const uint32_t s = offs[o];
 	const uint32_t len = offs[o + 1] - s;
 	const int L = len < (uint32_t)MAX_PREFIX ? (int)len : MAX_PREFIX;
+	// the (at most 18) prefix residues with aligned dword loads: six words
+	// cover 18 bytes at any alignment, and the residue buffer carries 16
+	// bytes of slack past its end
+	const uintptr_t addr = (uintptr_t)(chars + s);
+	const uint32_t* base = (const uint32_t*)(addr & ~(uintptr_t)3);
+	const uint32_t ash = (uint32_t)(addr & 3);
+	const int nw = L > 0 ? (int)((ash + (uint32_t)L - 1) >> 2) + 1 : 0;
+	uint32_t raw[6];
+#pragma unroll
+	for (int k = 0; k < 6; ++k)
+		raw[k] = k < nw ? base[k] : 0u;
+	uint32_t wd[5];
+#pragma unroll
+	for (int k = 0; k < 5; ++k)
+		wd[k] = __builtin_amdgcn_alignbyte(raw[k + 1], raw[k], ash);
 	uint64_t lo = 0, hi = 0;
-	for (int k = 0; k < L; ++k) {
-		const uint64_t code = (uint64_t)(chars[s + k] - '@') & 0x1Full;
+#pragma unroll
+	for (int k = 0; k < MAX_PREFIX; ++k) {
+		const uint32_t c = (wd[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+		const uint64_t code = k < L ? (uint64_t)((c - (uint32_t)'@') & 0x1Fu) : 0ull;
 		const int sh = 5 * (MAX_PREFIX - 1 - k);
 		if (sh >= 64)
 			hi |= code << (sh - 64);
