@@ -1030,7 +1030,9 @@ constexpr int BK_CAP = 3072;             // largest bucket grouped in LDS
 constexpr int BK_PER = BK_CAP / BK_NT;   // records per thread
 constexpr uint16_t BK_NONE = 0xFFFF;
 
-constexpr int BK_STAGE = 384;  // pairs staged in LDS between flushes
+constexpr int BK_STAGE = 376;  // pairs staged in LDS between flushes
+constexpr int BK_WAVES = BK_NT / WAVE;
+constexpr int BK_GROUP = 3;     // record slots per walk pass: a wave lists at most 3 * 64 walkers
 
 // Buckets of <= BK_CAP records, each handled by one workgroup in LDS: every
 // record is pushed on the list of its bin (12 more hash bits, one atomicExch
@@ -1064,9 +1066,12 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	__shared__ uint16_t s_next[BK_CAP];
 	__shared__ uint32_t s_head[BK_BINS];  // (iteration << 16) | record of the bin's last push
 	__shared__ uint64_t s_stage[BK_STAGE];
+	__shared__ uint16_t s_walk[BK_WAVES][BK_GROUP * WAVE];  // each wave's records that have a predecessor
 	__shared__ uint32_t s_nstage;
 	__shared__ unsigned long long s_base;
 	const int t = threadIdx.x;
+	const int lane = t & (WAVE - 1);
+	const int w = t / WAVE;
 	const uint32_t bmask = (1u << a.bin_bits) - 1u;
 	if (t == 0)
 		s_nstage = 0;
@@ -1121,26 +1126,50 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 					pv[i / 2] = link;
 			}
 			__syncthreads();
+			// Only records with a predecessor in their bin walk (about one in
+			// five): each wave lists its walkers of BK_GROUP slots in its own
+			// LDS row and walks the list densely, a lane per walker, instead
+			// of running one walk per slot with most lanes idle. The row is
+			// the wave's own, so no barrier guards it.
 #pragma unroll
-			for (int i = 0; i < BK_PER; ++i) {
-				const int j = i * BK_NT + t;
-				if (j >= cnt)
-					continue;
-				for (uint32_t n = (pv[i / 2] >> (16 * (i & 1))) & 0xFFFFu; n != BK_NONE;) {
-					const uint64_t kn = s_keys[n];  // key and link read together
-					const uint32_t nn = s_next[n];
-					uint64_t p;
-					const bool ok = pair_of(k[i], kn, a.idb, &p);
-					n = nn;
-					if (!ok)
-						continue;
-					const uint32_t slot = atomicAdd(&s_nstage, 1u);
-					if (slot < (uint32_t)BK_STAGE) {
-						s_stage[slot] = p;
-					} else {  // stage full: straight to the global buffer
-						const unsigned long long g = atomicAdd(&a.ctr->n_cand, 1ull);
-						if ((int64_t)g < a.pcap)
-							a.pairs[g] = p;
+			for (int g = 0; g < BK_PER; g += BK_GROUP) {
+				if (g * BK_NT >= cnt)
+					break;  // uniform
+				uint32_t nw = 0;  // walkers listed, uniform over the wave
+#pragma unroll
+				for (int i = g; i < g + BK_GROUP && i < BK_PER; ++i) {
+					const int j = i * BK_NT + t;
+					const bool walker = j < cnt && ((pv[i / 2] >> (16 * (i & 1))) & 0xFFFFu) != BK_NONE;
+					const uint64_t m = __ballot(walker);
+					if (walker)
+						s_walk[w][nw + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+						                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = (uint16_t)j;
+					nw += (uint32_t)__popcll(m);
+				}
+				for (uint32_t q0 = 0; q0 < nw; q0 += WAVE) {  // uniform
+					uint32_t n = BK_NONE;
+					uint64_t kj = 0;
+					if (q0 + lane < nw) {
+						const uint32_t j = s_walk[w][q0 + lane];
+						kj = s_keys[j];
+						n = s_next[j];
+					}
+					for (; n != BK_NONE;) {
+						const uint64_t kn = s_keys[n];  // key and link read together
+						const uint32_t nn = s_next[n];
+						uint64_t p;
+						const bool ok = pair_of(kj, kn, a.idb, &p);
+						n = nn;
+						if (!ok)
+							continue;
+						const uint32_t slot = atomicAdd(&s_nstage, 1u);
+						if (slot < (uint32_t)BK_STAGE) {
+							s_stage[slot] = p;
+						} else {  // stage full: straight to the global buffer
+							const unsigned long long gs = atomicAdd(&a.ctr->n_cand, 1ull);
+							if ((int64_t)gs < a.pcap)
+								a.pairs[gs] = p;
+						}
 					}
 				}
 			}
