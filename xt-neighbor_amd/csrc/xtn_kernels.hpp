@@ -1127,25 +1127,12 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 			}
 			__syncthreads();
 			// Only records with a predecessor in their bin walk (about one in
-			// five): each wave lists its walkers of BK_GROUP slots in its own
-			// LDS row and walks the list densely, a lane per walker, instead
-			// of running one walk per slot with most lanes idle. The row is
-			// the wave's own, so no barrier guards it.
-#pragma unroll
-			for (int g = 0; g < BK_PER; g += BK_GROUP) {
-				if (g * BK_NT >= cnt)
-					break;  // uniform
-				uint32_t nw = 0;  // walkers listed, uniform over the wave
-#pragma unroll
-				for (int i = g; i < g + BK_GROUP && i < BK_PER; ++i) {
-					const int j = i * BK_NT + t;
-					const bool walker = j < cnt && ((pv[i / 2] >> (16 * (i & 1))) & 0xFFFFu) != BK_NONE;
-					const uint64_t m = __ballot(walker);
-					if (walker)
-						s_walk[w][nw + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-						                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = (uint16_t)j;
-					nw += (uint32_t)__popcll(m);
-				}
+			// five): each wave lists its walkers in its own LDS row and walks
+			// the list densely, a lane per walker, instead of running one walk
+			// per record slot with most lanes idle. A group of BK_GROUP slots
+			// joins the list when it fits, else the list is walked first. The
+			// row is the wave's own, so no barrier guards it.
+			auto walk = [&](uint32_t nw) {
 				for (uint32_t q0 = 0; q0 < nw; q0 += WAVE) {  // uniform
 					uint32_t n = BK_NONE;
 					uint64_t kj = 0;
@@ -1172,7 +1159,35 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 						}
 					}
 				}
+			};
+			uint32_t nw = 0;  // walkers listed, uniform over the wave
+#pragma unroll
+			for (int g = 0; g < BK_PER; g += BK_GROUP) {
+				if (g * BK_NT >= cnt)
+					break;  // uniform
+				uint64_t m[BK_GROUP];
+				uint32_t c = 0;
+#pragma unroll
+				for (int u = 0; u < BK_GROUP; ++u) {
+					const int i = g + u;
+					const int j = i * BK_NT + t;
+					m[u] = __ballot(i < BK_PER && j < cnt && ((pv[i / 2] >> (16 * (i & 1))) & 0xFFFFu) != BK_NONE);
+					c += (uint32_t)__popcll(m[u]);
+				}
+				if (nw + c > (uint32_t)(BK_GROUP * WAVE)) {
+					walk(nw);
+					nw = 0;
+				}
+#pragma unroll
+				for (int u = 0; u < BK_GROUP; ++u) {
+					if ((m[u] >> lane) & 1u)
+						s_walk[w][nw + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[u] >> 32),
+						                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m[u], 0u))] =
+						    (uint16_t)((g + u) * BK_NT + t);
+					nw += (uint32_t)__popcll(m[u]);
+				}
 			}
+			walk(nw);
 			__syncthreads();
 		}
 		const uint32_t ns = s_nstage;
