@@ -441,7 +441,7 @@ __device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_vali
 	rank_tile_n<RS_IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 }
 
-template <bool KV, int NT = RS_NT, bool STABLE = true>
+template <bool KV, int NT = RS_NT>
 __global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
                                                          uint64_t* __restrict__ keys_out,
@@ -478,10 +478,7 @@ __global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict
 	if (t < RS_BINS)
 		s_gbase[t] = offs[tile * RS_BINS + t];
 	uint32_t lp[IT];
-	if constexpr (STABLE)
-		rank_tile_g<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
-	else
-		rank_tile_u<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+	rank_tile_g<NT, IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 #pragma unroll
 	for (int r = 0; r < IT; ++r) {
 		if (lp[r] != 0xFFFFFFFFu) {
