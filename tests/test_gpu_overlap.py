@@ -30,6 +30,23 @@ def run(engine, seqs=None, chars=None, offsets=None, freq=None, rep=None, d=1, m
     return out
 
 
+def test_bad_repertoire_indexes(engine):
+    # checked on the device: the first row with a negative repertoire index,
+    # and more repertoires than the dense R x R matrix holds
+    seqs = ["CASSLGF", "CASSLGW", "CASSPGF", "CATSLGF"]
+    engine.configure(distance=1, measure=0, max_records_per_chunk=0)
+    engine.set_sequences(seqs)
+    with pytest.raises(xtn.XtnError) as e:
+        engine.overlap([1, 2, 3, 4], [0, 1, -2, -1])
+    assert e.value.status == 3 and "row 2" in str(e.value)
+    with pytest.raises(xtn.XtnError) as e:
+        engine.overlap([1, 1, 1, 1], [0, 46340, 1, 2])
+    assert e.value.status == 6 and "46341 repertoires" in str(e.value)
+    out = engine.overlap([1, 1, 1, 1], [0, 1999, 1, 2])  # the handle still works
+    want = oracle.overlap(*xtn.encode(seqs), [1, 1, 1, 1], [0, 1999, 1, 2], 1, 0)
+    assert out.tolist() == want.tolist()
+
+
 def test_reference_vectors(engine, golden):
     for key in ("overlap_init", "stream4_overlap"):
         g = golden[key]

@@ -908,27 +908,24 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 	const int32_t n = h->n;
 	xtn_stats& st = h->stats;
 	const double t0 = now_ms();
-	std::vector<int32_t> hfr((size_t)std::max(n, 1) * 2);
-	int32_t R = 0;
-	for (int32_t i = 0; i < n; ++i) {
-		if (info[i].repertoire < 0)
-			return fail(h, XTN_ERR_ARG, "negative repertoire index at row %d", i);
-		hfr[i] = info[i].frequency;
-		hfr[(size_t)n + i] = info[i].repertoire;
-		R = std::max(R, info[i].repertoire + 1);
-	}
-	const int64_t RR = (int64_t)R * R;
-	if (RR > ((int64_t)1 << 31))
-		return fail(h, XTN_ERR_CAPACITY, "%d repertoires exceed the dense overlap matrix limit (46340)", R);
 	if (n == 0) {
 		xtn_output out{nullptr, nullptr, nullptr, 0};
 		if (callback)
 			callback(user, &out);
 		return XTN_OK;
 	}
+	// the rows go up as the caller holds them; the device splits them and
+	// finds the repertoire count and any negative repertoire index, read
+	// back with the unique count below
+	static_assert(sizeof(xtn_seq_info) == 3 * sizeof(int32_t), "packed SeqInfo rows");
 	TRY(ensure(h, h->ov_info, sizeof(int32_t) * 2 * (size_t)n));
-	HIP_TRY(h, hipMemcpyAsync(h->ov_info.p, hfr.data(), sizeof(int32_t) * 2 * (size_t)n,
-	                          hipMemcpyHostToDevice, s));
+	TRY(ensure(h, h->ov_keys_alt, sizeof(xtn_seq_info) * (size_t)n));  // staging, reused below
+	TRY(ensure(h, h->scalar, sizeof(uint64_t)));
+	HIP_TRY(h, hipMemcpyAsync(h->ov_keys_alt.p, info, sizeof(xtn_seq_info) * (size_t)n, hipMemcpyHostToDevice, s));
+	k_fill_u64<<<1, 256, 0, s>>>(h->scalar.as<unsigned long long>(), 1, 0xFFFFFFFF00000000ull);
+	k_split_info<<<grid_for(n, 256), 256, 0, s>>>(h->ov_keys_alt.as<int32_t>(), n, h->ov_info.as<int32_t>(),
+	                                               h->ov_info.as<int32_t>() + n, h->scalar.as<uint32_t>());
+	HIP_LAUNCH_CHECK(h);
 	const int32_t* freq_d = h->ov_info.as<int32_t>();
 	const int32_t* rep_d = freq_d + n;
 
@@ -964,9 +961,16 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 	k_group_flags<<<grid_for(n, 256), 256, 0, s>>>(lead, n, h->flags.as<uint32_t>());
 	HIP_LAUNCH_CHECK(h);
 	TRY(scan_excl<uint32_t>(h, h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), n, h->pos.as<uint32_t>() + n));
-	uint32_t U = 0;
+	uint32_t U = 0, ctl[2] = {0, 0};
 	HIP_TRY(h, hipMemcpyAsync(&U, h->pos.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_TRY(h, hipMemcpyAsync(ctl, h->scalar.p, sizeof(ctl), hipMemcpyDeviceToHost, s));
 	HIP_TRY(h, hipStreamSynchronize(s));
+	if (ctl[1] != 0xFFFFFFFFu)
+		return fail(h, XTN_ERR_ARG, "negative repertoire index at row %u", ctl[1]);
+	const int32_t R = (int32_t)ctl[0];
+	const int64_t RR = (int64_t)R * R;
+	if (RR > ((int64_t)1 << 31))
+		return fail(h, XTN_ERR_CAPACITY, "%d repertoires exceed the dense overlap matrix limit (46340)", R);
 	TRY(ensure(h, h->ov_ustart, sizeof(uint32_t) * ((size_t)U + 1)));
 	TRY(ensure(h, h->ov_umap, sizeof(uint32_t) * (size_t)U));
 	k_group_starts<<<grid_for(n, 256), 256, 0, s>>>(h->flags.as<uint32_t>(), h->pos.as<uint32_t>(), rows, n,

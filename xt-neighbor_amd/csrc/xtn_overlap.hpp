@@ -31,6 +31,38 @@ namespace xtn {
 
 constexpr int OV_LDS_REPS = 64;  // R <= 64: per-workgroup LDS copy of the matrix
 
+// The caller's rows ({frequency, repertoire, original_index}, 12 bytes each)
+// split into a frequency and a repertoire array, with the repertoire count
+// (largest index + 1) into ctl[0] and the first row holding a negative
+// repertoire index into ctl[1] (ctl zeroed / all ones beforehand); one
+// atomic per wave for each.
+__global__ __launch_bounds__(256) void k_split_info(const int32_t* __restrict__ rows, int32_t n,
+                                                    int32_t* __restrict__ freq, int32_t* __restrict__ rep,
+                                                    uint32_t* __restrict__ ctl) {
+	const int32_t i = blockIdx.x * 256 + threadIdx.x;
+	int32_t r = 0;
+	if (i < n) {
+		freq[i] = rows[3 * (int64_t)i];
+		r = rows[3 * (int64_t)i + 1];
+		rep[i] = r;
+	}
+	uint32_t top = i < n && r >= 0 ? (uint32_t)r + 1u : 0u;
+	uint32_t bad = i < n && r < 0 ? (uint32_t)i : 0xFFFFFFFFu;
+#pragma unroll
+	for (int o = WAVE / 2; o > 0; o >>= 1) {
+		const uint32_t t2 = (uint32_t)__shfl_xor((int)top, o);
+		const uint32_t b2 = (uint32_t)__shfl_xor((int)bad, o);
+		top = t2 > top ? t2 : top;
+		bad = b2 < bad ? b2 : bad;
+	}
+	if ((threadIdx.x & (WAVE - 1)) == 0) {
+		if (top)
+			atomicMax(&ctl[0], top);
+		if (bad != 0xFFFFFFFFu)
+			atomicMin(&ctl[1], bad);
+	}
+}
+
 __device__ __forceinline__ uint64_t hash_bytes(const uint8_t* __restrict__ s, uint32_t len) {
 	uint64_t h = 0xCBF29CE484222325ull ^ ((uint64_t)len * 0x9E3779B97F4A7C15ull);
 	for (uint32_t i = 0; i < len; ++i)
