@@ -94,6 +94,25 @@ _lib = None
 _synth = None
 
 
+_HIP = None
+
+
+def _hip_runtime():
+    """the HIP runtime library (for pinned host buffers), or None"""
+    global _HIP
+    if _HIP is None:
+        for name in ("libamdhip64.so", "libamdhip64.so.7"):
+            try:
+                _HIP = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _HIP is not None:
+            _HIP.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+            _HIP.hipHostFree.argtypes = [ctypes.c_void_p]
+    return _HIP
+
+
 def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     """Loads libxtn.so; raises when it has not been built."""
     global _lib
@@ -289,6 +308,26 @@ class Engine:
         if st != 0:
             raise XtnError(st, "xtn_open failed (no GPU or invalid configuration)")
         self._keep = None
+        self._pin = ctypes.c_void_p()
+        self._pin_bytes = 0
+
+    def _staging(self, nbytes: int):
+        """a pinned host buffer of at least nbytes, kept for later calls: the
+        library's uploads from it skip the runtime's pageable staging copy
+        (None when the runtime cannot allocate one)"""
+        if nbytes > self._pin_bytes:
+            self._free_staging()
+            hip = _hip_runtime()
+            p = ctypes.c_void_p()
+            if hip is None or hip.hipHostMalloc(ctypes.byref(p), nbytes, 0) != 0 or not p.value:
+                return None
+            self._pin, self._pin_bytes = p, nbytes
+        return self._pin
+
+    def _free_staging(self):
+        if self._pin.value:
+            _hip_runtime().hipHostFree(self._pin)
+        self._pin, self._pin_bytes = ctypes.c_void_p(), 0
 
     def _check(self, st: int):
         if st != 0:
@@ -298,6 +337,7 @@ class Engine:
         if self.h:
             self.lib.xtn_close(self.h)
             self.h = ctypes.c_void_p()
+        self._free_staging()
 
     def __enter__(self):
         return self
@@ -372,8 +412,15 @@ class Engine:
         freq = np.asarray(freq, dtype=np.int32)
         rep = np.asarray(rep, dtype=np.int32)
         n = len(freq)
-        rows = np.zeros(max(n, 1), dtype=[("frequency", "<i4"), ("repertoire", "<i4"),
-                                           ("original_index", "<i4")])
+        dt = np.dtype([("frequency", "<i4"), ("repertoire", "<i4"), ("original_index", "<i4")])
+        m = max(n, 1)
+        pin = self._staging(m * dt.itemsize)  # rows built in pinned memory when possible
+        if pin is not None:
+            rows = np.ctypeslib.as_array(ctypes.cast(pin, ctypes.POINTER(ctypes.c_uint8)),
+                                         shape=(m * dt.itemsize,)).view(dt)
+            rows[n:] = 0
+        else:
+            rows = np.zeros(m, dtype=dt)
         rows["frequency"][:n] = freq
         rows["repertoire"][:n] = rep
         rows["original_index"][:n] = np.arange(n, dtype=np.int32)
