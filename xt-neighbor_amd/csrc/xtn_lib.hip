@@ -326,7 +326,7 @@ int scan_excl(xtn_handle* h, const T* in, T* out, int64_t n, T* d_total) {
 	ProfScope ps(h, "scan", 3.0 * sizeof(T) * (double)n);
 	k_scan_sums<T><<<grid_for(nt, 1), SC_NT, 0, s>>>(in, n, tmp);
 	HIP_LAUNCH_CHECK(h);
-	k_scan_single<T><<<1, SC_NT, 0, s>>>(tmp, nt, d_total);
+	k_scan_single<T><<<1, SC_NT, 0, s>>>(tmp, nt, d_total, nullptr, 1);
 	HIP_LAUNCH_CHECK(h);
 	k_scan_apply<T><<<grid_for(nt, 1), SC_NT, 0, s>>>(in, out, n, tmp);
 	HIP_LAUNCH_CHECK(h);
@@ -836,7 +836,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ProfScope ps(h, "unique", (double)n_pairs * 16.0);
 			k_uniq_count<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt);
 			HIP_LAUNCH_CHECK(h);
-			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(ucnt, nt, ucnt + nt);
+			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(ucnt, nt, ucnt + nt, nullptr, 1);
 			HIP_LAUNCH_CHECK(h);
 			k_uniq_write<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt, upairs);
 			HIP_LAUNCH_CHECK(h);
@@ -848,7 +848,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			    upairs, ucnt + nt, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), seq_map, row_base,
 			    set_of, d, measure, h->dist.as<int8_t>(), kcnt);
 			HIP_LAUNCH_CHECK(h);
-			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(kcnt, nv, kcnt + nv);
+			// only the tiles holding unique pairs have counts to scan
+			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(kcnt, nv, kcnt + nv, ucnt + nt, SEL_NT);
 			HIP_LAUNCH_CHECK(h);
 		}
 		// the outputs hold every unique pair (at most n_pairs), so the select

@@ -137,11 +137,18 @@ __global__ __launch_bounds__(SC_NT) void k_scan_sums(const T* __restrict__ in, i
 		sums[blockIdx.x] = tot;
 }
 
-// exclusive scan of data[0..n) in place by a single workgroup
+// exclusive scan of data[0..n) in place by a single workgroup, the total into
+// *total; with n_dev, n shrinks to ceil(*n_dev / per) (entries past it are
+// zero and are left as they are)
 template <typename T>
 __global__ __launch_bounds__(SC_NT) void k_scan_single(T* __restrict__ data, int64_t n,
-                                                       T* __restrict__ total) {
+                                                       T* __restrict__ total,
+                                                       const uint32_t* __restrict__ n_dev, int per) {
 	__shared__ T s_tmp[SC_NT / WAVE + 1];
+	if (n_dev) {  // only the first ceil(*n_dev / per) entries can be nonzero
+		const int64_t m = ((int64_t)*n_dev + per - 1) / per;
+		n = m < n ? m : n;
+	}
 	T carry = 0;
 	for (int64_t base = 0; base < n; base += SC_TILE) {
 		T vals[SC_IT];
