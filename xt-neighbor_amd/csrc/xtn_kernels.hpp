@@ -681,7 +681,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			}
 			if (MODE == GEN_DIRECT) {
 				if (valid) {
-					a.out[r] = rec;
+					__builtin_nontemporal_store(rec, &a.out[r]);  // whole lines, read only by the next pass
 					if (a.digit_hist)
 						atomicAdd(&s_dig[(uint32_t)(rec >> a.digit_shift) & 0xFFu], 1u);
 				}
@@ -967,7 +967,7 @@ __global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
 #pragma unroll
 	for (int r = 0; r < IT; ++r) {
 		const bool valid = r * WAVE + lane < w_valid;
-		k[r] = valid ? a.in[seg + (int64_t)r * WAVE + lane] : 0ull;
+		k[r] = valid ? __builtin_nontemporal_load(&a.in[seg + (int64_t)r * WAVE + lane]) : 0ull;  // read once
 	}
 	uint32_t lp[IT];  // slot order inside a bucket is free: the unstable rank
 	rank_tile_u<NT, IT>(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);
@@ -1057,7 +1057,7 @@ __device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t
 #pragma unroll
 	for (int i = 0; i < BK_PER; ++i) {
 		const int j = i * BK_NT + (int)threadIdx.x;
-		k[i] = (c <= a.cap && j < c) ? a.rec[*s + j] : 0ull;
+		k[i] = (c <= a.cap && j < c) ? __builtin_nontemporal_load(&a.rec[*s + j]) : 0ull;  // read once
 	}
 }
 

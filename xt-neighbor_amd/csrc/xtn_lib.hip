@@ -759,7 +759,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			HIP_LAUNCH_CHECK(h);
 			{
 				ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)nrec);
-				k_radix_scatter<false><<<(unsigned)rtiles, RS_NT, 0, s>>>(
+				k_radix_scatter<false, RS_NT, true><<<(unsigned)rtiles, RS_NT, 0, s>>>(
 				    h->recs.as<uint64_t>(), nullptr, h->recs_alt.as<uint64_t>(), nullptr, nrec, hi - 8, hist, rtiles);
 				HIP_LAUNCH_CHECK(h);
 			}

@@ -448,7 +448,9 @@ __device__ __forceinline__ void rank_tile(const uint64_t (&k)[RS_IT], int w_vali
 	rank_tile_n<RS_IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
 }
 
-template <bool KV, int NT = RS_NT>
+// STREAM: the keys are read once and never again from this pass (the
+// record path's first pass), so the loads carry the non-temporal hint
+template <bool KV, int NT = RS_NT, bool STREAM = false>
 __global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict__ keys_in,
                                                          const uint32_t* __restrict__ vals_in,
                                                          uint64_t* __restrict__ keys_out,
@@ -478,7 +480,10 @@ __global__ __launch_bounds__(NT) void k_radix_scatter(const uint64_t* __restrict
 	for (int r = 0; r < IT; ++r) {
 		const bool valid = r * WAVE + lane < w_valid;
 		const int64_t idx = seg + (int64_t)r * WAVE + lane;
-		k[r] = valid ? keys_in[idx] : 0ull;
+		if constexpr (STREAM)
+			k[r] = valid ? __builtin_nontemporal_load(&keys_in[idx]) : 0ull;
+		else
+			k[r] = valid ? keys_in[idx] : 0ull;
 		if (KV)
 			v[r] = valid ? vals_in[idx] : 0u;
 	}
