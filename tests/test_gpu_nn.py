@@ -89,6 +89,19 @@ def test_hash_paths_agree_across_tiles(engine, d, base_len, ins):
     assert sum(1 for i, j, _ in got.tolist() if i < 3000 <= j) >= 1500
 
 
+@pytest.mark.parametrize("d", [1, 2])
+def test_short_strings_fill_tiles(engine, d):
+    # strings of 2-4 residues have 3-11 variants: a 4096-record generator
+    # tile then spans more than 448 sequences, the form that reads each
+    # record's packed prefix from global memory (mixed here with fuller
+    # tiles of longer strings)
+    rng = np.random.default_rng(70 + d)
+    seqs = random_seqs(rng, 2500, 2, 4) + random_seqs(rng, 1500, 8, 14, alphabet="ACDE")
+    chars, offsets = xtn.encode(seqs)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=d)
+    assert np.array_equal(got, oracle.nn_brute(chars, offsets, d, 0))
+
+
 @pytest.mark.parametrize("copies", [3, 4, 7])
 def test_runs_across_tile_boundaries(engine, copies):
     # every string repeated: thousands of runs of 3..10 records, many of
