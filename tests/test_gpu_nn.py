@@ -221,6 +221,19 @@ def test_invalid_residue_rejected(engine):
     with pytest.raises(xtn.XtnError) as e:
         engine.set_sequences(["CASS", "CAXS"])
     assert e.value.status == 1
+    # the residue check runs in 64 KiB blocks: a bad byte in the second block,
+    # each excluded letter and bytes either side of A..Y
+    res = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", np.uint8)
+    chars = res[np.arange(70_000) % 20].copy()
+    offsets = np.array([0, 40_000, 70_000], np.uint32)
+    for bad in b"BJOUX@Z[az\x00\xff":
+        c = chars.copy()
+        c[65_536] = bad
+        with pytest.raises(xtn.XtnError) as e:
+            engine.set_sequences(chars=c, offsets=offsets)
+        assert e.value.status == 1, bad
+    engine.set_sequences(chars=chars, offsets=offsets)
+    assert engine.nn()[0].shape == (0, 2)
 
 
 def test_cli_end_to_end(tmp_path):

@@ -882,10 +882,23 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	return XTN_OK;
 }
 
-bool valid_residue(unsigned char c) {
-	if (c < 'A' || c > 'Y')
-		return false;
-	return !(c == 'B' || c == 'J' || c == 'O' || c == 'U' || c == 'X');
+// every byte one of the 20 residues ACDEFGHIKLMNPQRSTVWY (codec.cu:25-38):
+// branch-free byte compares OR-ed over 64 KiB blocks, which the compiler
+// vectorises, with one test per block
+bool all_residues(const char* chars, size_t total) {
+	const unsigned char* c = (const unsigned char*)chars;
+	for (size_t b = 0; b < total; b += 65536) {
+		const size_t e = std::min(total, b + 65536);
+		uint8_t bad = 0;
+		for (size_t k = b; k < e; ++k) {
+			const uint8_t x = c[k];
+			bad |= (uint8_t)((uint8_t)(x - 'A') > (uint8_t)('Y' - 'A')) | (uint8_t)(x == 'B') |
+			       (uint8_t)(x == 'J') | (uint8_t)(x == 'O') | (uint8_t)(x == 'U') | (uint8_t)(x == 'X');
+		}
+		if (bad)
+			return false;
+	}
+	return true;
 }
 
 void print_verbose(xtn_handle* h, const char* mode) {
@@ -1149,9 +1162,8 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 		if (offsets[i + 1] < offsets[i])
 			return fail(h, XTN_ERR_ARG, "offsets must be non-decreasing (row %d)", i);
 	const uint32_t total = offsets[n];
-	for (uint32_t k = 0; k < total; ++k)
-		if (!valid_residue((unsigned char)chars[k]))
-			return fail(h, XTN_ERROR, "invalid char");
+	if (!all_residues(chars, total))
+		return fail(h, XTN_ERROR, "invalid char");
 	HIP_TRY(h, hipSetDevice(h->cfg.device));
 	TRY(ensure(h, h->chars, (size_t)total + 16));
 	TRY(ensure(h, h->offs, sizeof(uint32_t) * ((size_t)n + 1)));
@@ -1737,11 +1749,10 @@ int xtn_server_submit(xtn_server* sv, const char* chars, const uint32_t* offsets
 			return XTN_ERR_ARG;
 		}
 	const uint32_t total = offsets[n];
-	for (uint32_t k = 0; k < total; ++k)
-		if (!valid_residue((unsigned char)chars[k])) {
-			sv->err = "invalid char";
-			return XTN_ERROR;
-		}
+	if (!all_residues(chars, total)) {
+		sv->err = "invalid char";
+		return XTN_ERROR;
+	}
 	rq.chars.assign(chars, chars + total);
 	rq.offs.assign(offsets, offsets + n + 1);
 	for (int32_t i = 0; i < n; ++i)
