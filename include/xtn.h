@@ -146,7 +146,7 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 
 int xtn_get_stats(xtn_handle* h, xtn_stats* out);
 
-/* ---- served form (no reference counterpart; SURVEY's query-vs-index mode).
+/* ---- served form (no reference counterpart: query sets against a resident index).
  * A server keeps the sorted deletion index of n index sequences resident in
  * HBM and answers query sets against it: every (query row, index row) pair
  * within the distance. `slots` worker streams run admitted queries
