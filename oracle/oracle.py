@@ -70,6 +70,9 @@ def lib() -> ctypes.CDLL:
     L.xo_overlap_terms.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, i32p, i32p, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint64))]
     L.xo_overlap_terms.restype = ctypes.c_int64
+    L.xo_overlap_port_mt.argtypes = [ctypes.c_char_p, u32p, ctypes.c_int, i32p, i32p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.POINTER(ctypes.c_uint64))]
+    L.xo_overlap_port_mt.restype = ctypes.c_int64
     L.xo_free.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
@@ -217,6 +220,21 @@ def overlap(chars, offsets, freq, rep, d: int, measure: int = 0, which: int = 3)
                                f.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), d, measure, which,
                                ctypes.byref(out))
+    arr = np.ctypeslib.as_array(out, shape=(3 * m,)).copy() if m else np.zeros(0, np.uint64)
+    lib().xo_free(ctypes.cast(out, ctypes.c_void_p))
+    return arr.reshape(-1, 3)
+
+
+def overlap_mt(chars, offsets, freq, rep, d: int, measure: int = 0, threads: int = 1) -> np.ndarray:
+    """overlap() with the neighbour search on `threads` OpenMP threads"""
+    chars, offsets = _seq_args(chars, offsets)
+    f = np.ascontiguousarray(freq, dtype=np.int32)
+    r = np.ascontiguousarray(rep, dtype=np.int32)
+    out = ctypes.POINTER(ctypes.c_uint64)()
+    m = lib().xo_overlap_port_mt(chars.ctypes.data_as(ctypes.c_char_p), _p(offsets), len(offsets) - 1,
+                                 f.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                 r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), d, measure, threads,
+                                 ctypes.byref(out))
     arr = np.ctypeslib.as_array(out, shape=(3 * m,)).copy() if m else np.zeros(0, np.uint64)
     lib().xo_free(ctypes.cast(out, ctypes.c_void_p))
     return arr.reshape(-1, 3)

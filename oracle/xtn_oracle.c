@@ -839,7 +839,8 @@ static void rep_push(rep_list* l, int32_t a, int32_t b, uint64_t v) {
  * through the last member of each group, expand to repertoire pairs) and
  * mergeOutput (xtn_overlap_inner.cu:212-245: sort by (A, B), sum by key). */
 static int64_t overlap_terms(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,
-                             const int32_t* rep, int distance, int measure, int which, uint64_t** out) {
+                             const int32_t* rep, int distance, int measure, int which, int threads,
+                             uint64_t** out) {
 	str_ctx sc = {chars, offsets};
 	xo_info* info = (xo_info*)malloc(sizeof(xo_info) * (size_t)(n > 0 ? n : 1));
 	for (int i = 0; i < n; i++) {
@@ -884,7 +885,9 @@ static int64_t overlap_terms(const char* chars, const uint32_t* offsets, int n, 
 			uoff[g + 1] = uoff[g] + l;
 		}
 		int32_t* nn = NULL;
-		int64_t nnn = xo_nn_port(uchars, uoff, ng, distance, measure, 1, &nn, NULL);
+		/* threads > 1: the same NN output from the OpenMP port */
+		int64_t nnn = threads > 1 ? xo_nn_port_mt(uchars, uoff, ng, distance, measure, threads, &nn, NULL)
+		                          : xo_nn_port(uchars, uoff, ng, distance, measure, 1, &nn, NULL);
 		for (int64_t k = 0; k < nnn; k++) {
 			int x = nn[3 * k], y = nn[3 * k + 1];
 			for (int i = gstart[x]; i < gstart[x + 1]; i++)
@@ -923,12 +926,17 @@ static int64_t overlap_terms(const char* chars, const uint32_t* offsets, int n, 
 
 int64_t xo_overlap_port(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,
                         const int32_t* rep, int distance, int measure, uint64_t** out) {
-	return overlap_terms(chars, offsets, n, freq, rep, distance, measure, 3, out);
+	return overlap_terms(chars, offsets, n, freq, rep, distance, measure, 3, 1, out);
+}
+
+int64_t xo_overlap_port_mt(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,
+                           const int32_t* rep, int distance, int measure, int threads, uint64_t** out) {
+	return overlap_terms(chars, offsets, n, freq, rep, distance, measure, 3, threads, out);
 }
 
 int64_t xo_overlap_terms(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,
                          const int32_t* rep, int distance, int measure, int which, uint64_t** out) {
-	return overlap_terms(chars, offsets, n, freq, rep, distance, measure, which, out);
+	return overlap_terms(chars, offsets, n, freq, rep, distance, measure, which, 1, out);
 }
 
 void xo_free(void* p) {

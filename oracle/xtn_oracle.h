@@ -30,6 +30,7 @@
  *   xo_overlap_port     -> test_overlap_init.cu, test_stream4_overlap.cu,
  *                          data/dummy_input_airr.tsv expected matrix, and the
  *                          closed form R^T (A + I) R on duplicate-free input
+ *   xo_overlap_port_mt  -> xo_overlap_port at several thread counts
  */
 #ifndef XTN_ORACLE_H
 #define XTN_ORACLE_H
@@ -96,6 +97,10 @@ int64_t xo_neighbors_of(const char* chars, const uint32_t* offsets, int n, int r
  * row. *out receives 3 uint64 per entry (A, B, sum), sorted by (A, B). */
 int64_t xo_overlap_port(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,
                         const int32_t* rep, int distance, int measure, uint64_t** out);
+/* xo_overlap_port with the neighbour search on `threads` OpenMP threads
+ * (xo_nn_port_mt); same output. */
+int64_t xo_overlap_port_mt(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,
+                           const int32_t* rep, int distance, int measure, int threads, uint64_t** out);
 /* the same with a term selector: bit 0 = exact-match term, bit 1 = neighbour
  * term (both = xo_overlap_port). */
 int64_t xo_overlap_terms(const char* chars, const uint32_t* offsets, int n, const int32_t* freq,

@@ -80,3 +80,44 @@ def test_empty_and_invalid_queries(index):
         with pytest.raises(xtn.XtnError) as e:
             srv.submit(bad_c, bad_o)
         assert e.value.status == 1
+
+
+def test_ticket_collected_once_and_null_chars(index):
+    """a ticket is collected by one wait: a second wait on it, a wait on a
+    ticket never issued and a second concurrent waiter all get XTN_ERR_ARG
+    instead of blocking; NULL residues with a non-empty query are rejected"""
+    import ctypes
+    import threading
+
+    ichars, ioffs = index
+    with xtn.Server(ichars, ioffs, distance=1, slots=2) as srv:
+        qc, qo = xtn.synth_queries(7, ichars, ioffs, 300, planted=0.3)
+        t = srv.submit(qc, qo)
+        p, dist, _ = srv.wait(t)
+        assert np.array_equal(as_triples(p, dist), cross_pairs(ichars, ioffs, qc, qo, 1, 0))
+        for bad in (t, t + 1000, 0):
+            with pytest.raises(xtn.XtnError) as e:
+                srv.wait(bad)
+            assert e.value.status == 3
+        # two threads waiting on one ticket: one result, one error
+        t2 = srv.submit(qc, qo)
+        out = []
+
+        def w():
+            try:
+                out.append(("ok", srv.wait(t2)))
+            except xtn.XtnError as err:
+                out.append(("err", err.status))
+
+        ths = [threading.Thread(target=w) for _ in range(2)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join(timeout=60)
+        assert sorted(k for k, _ in out) == ["err", "ok"]
+        assert [v for k, v in out if k == "err"] == [3]
+        tk = ctypes.c_uint64()
+        st = srv.lib.xtn_server_submit(srv.s, None, xtn._u32p(np.ascontiguousarray(qo, np.uint32)),
+                                       len(qo) - 1, ctypes.byref(tk))
+        assert st == 3
+        assert b"invalid" in srv.lib.xtn_server_last_error(srv.s)

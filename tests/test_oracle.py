@@ -242,3 +242,22 @@ def test_port_mt_equals_port(threads):
         assert gst == wst
     empty, _ = oracle.nn_port_mt(*xtn.encode(["CASS"]), 1, 0, threads)
     assert len(empty) == 0
+
+
+@pytest.mark.parametrize("threads", [2, 8])
+def test_overlap_mt_equals_overlap(threads):
+    """the OpenMP overlap (the checker of the 64-repertoire workload) against
+    the single-thread restatement, with duplicates inside and across
+    repertoires and negative frequencies (sums that wrap mod 2^64)"""
+    rng = np.random.default_rng(40 + threads)
+    seqs = clustered_seqs(rng, 1200, 3, 22, "ACDEFGH", edits=1)
+    chars, offsets = xtn.encode(seqs)
+    n = len(seqs)
+    sizes = [300, 200, 400, 300]
+    rep = xtn.reps_from_sizes(sizes)
+    freq = rng.integers(-2**31 + 1, 2**31 - 1, n).astype(np.int32)
+    freq[freq == 0] = 1
+    for d, measure in [(1, 0), (2, 0), (1, 1)]:
+        want = oracle.overlap(chars, offsets, freq, rep, d, measure)
+        got = oracle.overlap_mt(chars, offsets, freq, rep, d, measure, threads)
+        assert np.array_equal(got, want)

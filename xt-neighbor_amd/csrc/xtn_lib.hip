@@ -21,6 +21,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <cstddef>
 #include <cstdarg>
 #include <cstdio>
@@ -1479,6 +1480,7 @@ struct xtn_server {
 	std::condition_variable cv_work, cv_done;
 	std::deque<ServeRequest> queue;
 	std::unordered_map<uint64_t, ServeResult> done;
+	std::unordered_set<uint64_t> open_tickets;  // issued and not yet claimed by a wait
 	uint64_t next_ticket = 1;
 	int64_t inflight = 0;
 	int64_t budget = 0;
@@ -1487,6 +1489,14 @@ struct xtn_server {
 };
 
 namespace {
+
+// the server's last error, written under its mutex (submit, wait and the
+// workers run on different threads)
+int sv_fail(xtn_server* sv, int code, const std::string& msg) {
+	std::lock_guard<std::mutex> lk(sv->mu);
+	sv->err = msg;
+	return code;
+}
 
 // one query on a worker handle: variants, index lookups, candidate sort and
 // unique, verification, compaction, download
@@ -1732,35 +1742,27 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 
 int xtn_server_submit(xtn_server* sv, const char* chars, const uint32_t* offsets, int32_t n, uint64_t* ticket) {
 	if (!sv || !offsets || n < 0 || !ticket)
-		return XTN_ERR_ARG;
-	if (n > sv->cfg.max_query_rows) {
-		sv->err = "query set larger than max_query_rows";
-		return XTN_ERR_CAPACITY;
-	}
-	if (offsets[0] != 0) {
-		sv->err = "offsets[0] must be 0";
-		return XTN_ERR_ARG;
-	}
+		return sv ? sv_fail(sv, XTN_ERR_ARG, "invalid query arguments") : XTN_ERR_ARG;
+	if (n > sv->cfg.max_query_rows)
+		return sv_fail(sv, XTN_ERR_CAPACITY, "query set larger than max_query_rows");
+	if (offsets[0] != 0)
+		return sv_fail(sv, XTN_ERR_ARG, "offsets[0] must be 0");
 	ServeRequest rq;
 	rq.n = n;
 	for (int32_t i = 0; i < n; ++i)
-		if (offsets[i + 1] < offsets[i]) {
-			sv->err = "offsets must be non-decreasing";
-			return XTN_ERR_ARG;
-		}
+		if (offsets[i + 1] < offsets[i])
+			return sv_fail(sv, XTN_ERR_ARG, "offsets must be non-decreasing");
 	const uint32_t total = offsets[n];
-	if (!all_residues(chars, total)) {
-		sv->err = "invalid char";
-		return XTN_ERROR;
-	}
+	if (total > 0 && !chars)
+		return sv_fail(sv, XTN_ERR_ARG, "invalid query arguments");
+	if (!all_residues(chars, total))
+		return sv_fail(sv, XTN_ERROR, "invalid char");
 	rq.chars.assign(chars, chars + total);
 	rq.offs.assign(offsets, offsets + n + 1);
 	for (int32_t i = 0; i < n; ++i)
 		rq.variants += (int64_t)host_variant_count(offsets[i + 1] - offsets[i], sv->cfg.distance);
-	if (rq.variants > (int64_t)INT32_MAX) {
-		sv->err = "query variants exceed 2^31";
-		return XTN_ERR_CAPACITY;
-	}
+	if (rq.variants > (int64_t)INT32_MAX)
+		return sv_fail(sv, XTN_ERR_CAPACITY, "query variants exceed 2^31");
 	// records, candidates (about V/2, sort buffer), flags/positions
 	rq.est_bytes = rq.variants * 8 + rq.variants * 12 + (int64_t)n * 64 + (int64_t)total;
 	rq.t_submit = now_ms();
@@ -1768,6 +1770,7 @@ int xtn_server_submit(xtn_server* sv, const char* chars, const uint32_t* offsets
 		std::lock_guard<std::mutex> lk(sv->mu);
 		rq.ticket = sv->next_ticket++;
 		*ticket = rq.ticket;
+		sv->open_tickets.insert(rq.ticket);
 		sv->queue.push_back(std::move(rq));
 	}
 	sv->cv_work.notify_all();
@@ -1780,21 +1783,22 @@ int xtn_server_wait(xtn_server* sv, uint64_t ticket, xtn_callback callback, void
 	ServeResult res;
 	{
 		std::unique_lock<std::mutex> lk(sv->mu);
-		if (ticket == 0 || ticket >= sv->next_ticket) {
-			sv->err = "unknown ticket";
+		// a ticket is collected by exactly one wait: a second wait on it (or
+		// one on a ticket never issued) is an error, not a wait forever
+		if (sv->open_tickets.erase(ticket) == 0) {
+			sv->err = "unknown or already collected ticket";
 			return XTN_ERR_ARG;
 		}
 		sv->cv_done.wait(lk, [&] { return sv->done.count(ticket) > 0; });
 		res = std::move(sv->done[ticket]);
 		sv->done.erase(ticket);
+		if (res.status != XTN_OK)
+			sv->err = res.err;
 	}
 	if (latency_ms)
 		*latency_ms = res.latency_ms;
-	if (res.status != XTN_OK) {
-		std::lock_guard<std::mutex> lk(sv->mu);
-		sv->err = res.err;
+	if (res.status != XTN_OK)
 		return res.status;
-	}
 	xtn_output out{res.pairs.data(), res.dist.data(), nullptr, (int64_t)res.pairs.size()};
 	if (callback)
 		callback(user, &out);
@@ -1817,6 +1821,15 @@ void xtn_server_close(xtn_server* sv) {
 	delete sv;
 }
 
-const char* xtn_server_last_error(xtn_server* sv) { return sv ? sv->err.c_str() : "null server"; }
+// a copy per calling thread, taken under the mutex: the text stays valid
+// until this thread's next call
+const char* xtn_server_last_error(xtn_server* sv) {
+	if (!sv)
+		return "null server";
+	static thread_local std::string copy;
+	std::lock_guard<std::mutex> lk(sv->mu);
+	copy = sv->err;
+	return copy.c_str();
+}
 
 }  // extern "C"
