@@ -116,7 +116,11 @@ def batched_vs_plain(eng, offsets, n_sets, seed):
         dt = time.perf_counter() - t0
         out[f"{name}_ms"] = dt * 1e3
         out[f"{name}_seqs_per_s"] = n / dt
-        out[f"{name}_output_pairs"] = eng.stats()["n_output"]
+        bst = eng.stats()
+        out[f"{name}_output_pairs"] = bst["n_output"]
+        out[f"{name}_launches"] = bst["n_launches"]
+        out[f"{name}_syncs"] = bst["n_syncs"]
+        out[f"{name}_graph_replays"] = bst["graph_replays"]
     out["sets"] = n_sets
     out["rows"] = n
     out["speedup"] = out["plain_ms"] / out["batched_ms"]
@@ -368,8 +372,8 @@ def main():
     profiled_ms = (time.perf_counter() - tp0) / args.steps * 1e3
     progress(f"{args.workload}: {step_s * 1e3:.3f} ms per step")
     kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "region", "bucket", "runs", "pairs_small", "pairs_big",
-               "sort_seg_hist", "sort_seg_scatter", "sort_pairs_hist", "sort_pairs_scatter", "unique", "verify",
-               "emit", "pack", "scan", "gen_count"]
+               "sort_seg_hist", "sort_seg_scatter", "tail_hist", "sort_pairs", "select", "pack", "scan",
+               "gen_count"]
     prof = {k: eng.kernel_profile(k) for k in kernels}
     dom = max(prof, key=lambda k: prof[k][1])
     launches, ms, byts = prof[dom]
@@ -396,6 +400,9 @@ def main():
                    "measure": "hamming" if measure else "leven", "planted": planted,
                    "max_edits": edits, "seed": seed},
         "verified_pairs_per_s": st["n_unique"] / step_s,
+        "launches_per_step": st["n_launches"],
+        "syncs_per_step": st["n_syncs"],
+        "graph_replay": bool(st["graph_replays"]),
         "output_pairs": out_len,
         "variants": st["n_variants"],
         "candidate_pairs": st["n_candidates"],

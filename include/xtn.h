@@ -86,6 +86,16 @@ typedef struct {
 	int32_t bucket_capacity;       /* records one workgroup groups in LDS
 	                                  (0 or above 3072: 3072); larger
 	                                  buckets take the global sort path */
+	int32_t no_graph;              /* 1: never capture or replay a run as a
+	                                  HIP graph (0: a single-chunk run is
+	                                  captured once it repeats and replayed
+	                                  while it keeps its sizes) */
+	int32_t reserved;
+	int64_t max_pairs_per_round;   /* 0: candidate pairs per round sized from
+	                                  free HBM (at most 2^30 - 1); > 0: cap
+	                                  them. A search with more candidates runs
+	                                  in rounds over ranges of the smaller id
+	                                  (xtn.cu:385-470), outputs in order */
 } xtn_config;
 
 typedef struct xtn_handle xtn_handle;
@@ -102,6 +112,13 @@ typedef struct {
 	int64_t n_output;      /* pairs (NN) or repertoire entries (overlap) out */
 	double ms_total;       /* device pipeline wall time */
 	double ms_generate, ms_sort, ms_pairs, ms_verify, ms_output;
+	int64_t n_rounds;      /* candidate rounds (ranges of the smaller id) */
+	int64_t tail_checked;  /* 1: the candidate tail ran with the host checking
+	                          each step (after a device-side wait hit its
+	                          limit on this handle) */
+	int64_t n_launches;    /* kernel launches of the last call */
+	int64_t n_syncs;       /* host waits on the device of the last call */
+	int64_t graph_replays; /* runs issued as one captured graph launch */
 } xtn_stats;
 
 /* fills *cfg with defaults: distance 1, Levenshtein, device 0 */

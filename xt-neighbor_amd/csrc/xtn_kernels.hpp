@@ -807,13 +807,21 @@ __global__ __launch_bounds__(RN_NT) void k_runs(const uint64_t* __restrict__ rec
 	}
 }
 
-__device__ __forceinline__ bool pair_of(uint64_t ka, uint64_t kb, int idb, uint64_t* out) {
-	if ((ka >> idb) != (kb >> idb))
+// the candidate pair of two records holding the same stored hash bits and two
+// different ids, whose smaller id lies in [lo, hi) (a lower-bound round,
+// xtn_inner.cu:245-267 / kernel.cu:501-521, restricts pairs the same way)
+template <bool RANGED = true>
+__device__ __forceinline__ bool pair_of(uint64_t ka, uint64_t kb, int idb, uint32_t lo, uint32_t hi,
+                                        uint64_t* out) {
+	if ((ka ^ kb) >> idb)
 		return false;  // same sorted bits, different variant hash
 	const uint64_t mask = (1ull << idb) - 1ull;
 	const uint64_t ia = ka & mask, ib = kb & mask;
 	if (ia == ib)
 		return false;  // a sequence paired with itself (kernel.cu:377-380 drops these later)
+	const uint64_t mi = ia < ib ? ia : ib;
+	if (RANGED && (mi < lo || mi >= hi))
+		return false;
 	*out = ia < ib ? ((ia << idb) | ib) : ((ib << idb) | ia);
 	return true;
 }
@@ -822,7 +830,8 @@ __device__ __forceinline__ bool pair_of(uint64_t ka, uint64_t kb, int idb, uint6
 // every group, one thread per group)
 __global__ __launch_bounds__(256) void k_pairs_small(const uint64_t* __restrict__ rec,
                                                      const Run* __restrict__ runs, int64_t nruns,
-                                                     int idb, uint64_t* __restrict__ pairs,
+                                                     int idb, uint32_t lo, uint32_t hi,
+                                                     uint64_t* __restrict__ pairs,
                                                      int64_t cap, RunCtrs* __restrict__ ctr) {
 	const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
 	Run run{0, 0};
@@ -833,7 +842,7 @@ __global__ __launch_bounds__(256) void k_pairs_small(const uint64_t* __restrict_
 		const uint64_t ka = rec[run.start + a];
 		for (uint32_t b = a + 1; b < run.len; ++b) {
 			uint64_t p;
-			count += pair_of(ka, rec[run.start + b], idb, &p) ? 1u : 0u;
+			count += pair_of(ka, rec[run.start + b], idb, lo, hi, &p) ? 1u : 0u;
 		}
 	}
 	__shared__ uint32_t s_tmp[256 / WAVE + 1];
@@ -843,7 +852,7 @@ __global__ __launch_bounds__(256) void k_pairs_small(const uint64_t* __restrict_
 		const uint64_t ka = rec[run.start + a];
 		for (uint32_t b = a + 1; b < run.len; ++b) {
 			uint64_t p;
-			if (pair_of(ka, rec[run.start + b], idb, &p)) {
+			if (pair_of(ka, rec[run.start + b], idb, lo, hi, &p)) {
 				if ((int64_t)slot < cap)
 					pairs[slot] = p;
 				++slot;
@@ -1005,6 +1014,7 @@ struct BucketArgs {
 	int32_t idb;
 	int32_t cap;        // largest bucket handled in LDS (<= BK_CAP)
 	int32_t ovf_cap;
+	uint32_t id_lo, id_hi;  // only pairs whose smaller id lies in [id_lo, id_hi)
 	uint64_t* pairs;
 	int64_t pcap;
 	RunCtrs* ctr;
@@ -1061,6 +1071,9 @@ __device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t
 	}
 }
 
+// RANGED: the round covers only part of the smaller ids (the id test is
+// compiled out of the common single-round case)
+template <bool RANGED>
 __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	__shared__ uint64_t s_keys[BK_CAP];
 	__shared__ uint16_t s_next[BK_CAP];
@@ -1080,25 +1093,10 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 	// a workgroup handles far fewer than 0xFFFF buckets
 	for (int i = t; i < BK_BINS; i += BK_NT)
 		s_head[i] = 0xFFFF0000u;
-	// two buckets in flight ahead of the one being binned
-	uint64_t k1[BK_PER], k2[BK_PER];
-	uint32_t s1, e1, s2, e2;
 	const int G = (int)gridDim.x;
-	bucket_load(a, blockIdx.x, k1, &s1, &e1);
-	bucket_load(a, blockIdx.x + G, k2, &s2, &e2);
-	__syncthreads();
-	uint32_t iter = 0;
-	for (int b = blockIdx.x; b < a.nb; b += G, ++iter) {  // uniform over the workgroup
-		uint64_t k[BK_PER];
-#pragma unroll
-		for (int i = 0; i < BK_PER; ++i) {
-			k[i] = k1[i];
-			k1[i] = k2[i];
-		}
-		const int cnt = (int)(e1 - s1);
-		s1 = s2;
-		e1 = e2;
-		bucket_load(a, b + 2 * G, k2, &s2, &e2);
+
+	// bins, walks and flushes bucket b (records in k, cnt of them)
+	auto process = [&](int b, uint32_t iter, const uint64_t (&k)[BK_PER], int cnt) {
 		if (cnt > a.cap) {
 			if (t == 0) {
 				const unsigned long long slot = atomicAdd(a.n_ovf, 1ull);
@@ -1127,11 +1125,11 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 			}
 			__syncthreads();
 			// Only records with a predecessor in their bin walk (about one in
-			// five): each wave lists its walkers in its own LDS row and walks
-			// the list densely, a lane per walker, instead of running one walk
-			// per record slot with most lanes idle. A group of BK_GROUP slots
-			// joins the list when it fits, else the list is walked first. The
-			// row is the wave's own, so no barrier guards it.
+			// five): each wave lists them in its own LDS row and walks the
+			// list densely, a lane per walker, instead of running one walk per
+			// record slot with most lanes idle. A group of BK_GROUP slots joins
+			// the list when it fits, else the list is walked first. The row is
+			// the wave's own, so no barrier guards it.
 			auto walk = [&](uint32_t nw) {
 				for (uint32_t q0 = 0; q0 < nw; q0 += WAVE) {  // uniform
 					uint32_t n = BK_NONE;
@@ -1145,15 +1143,23 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 						const uint64_t kn = s_keys[n];  // key and link read together
 						const uint32_t nn = s_next[n];
 						uint64_t p;
-						const bool ok = pair_of(kj, kn, a.idb, &p);
+						const bool ok = pair_of<RANGED>(kj, kn, a.idb, a.id_lo, a.id_hi, &p);
 						n = nn;
 						if (!ok)
 							continue;
 						const uint32_t slot = atomicAdd(&s_nstage, 1u);
-						if (slot < (uint32_t)BK_STAGE) {
+						const bool spill = slot >= (uint32_t)BK_STAGE;
+						if (!spill) {
 							s_stage[slot] = p;
-						} else {  // stage full: straight to the global buffer
-							const unsigned long long gs = atomicAdd(&a.ctr->n_cand, 1ull);
+						} else {
+							// stage full (a large group): straight to the global
+							// buffer, one atomic for the wave's spilling lanes
+							const uint64_t m = __ballot(spill);
+							const int leader = __ffsll((unsigned long long)m) - 1;
+							unsigned long long gs = 0;
+							if (lane == leader)
+								gs = atomicAdd(&a.ctr->n_cand, (unsigned long long)__popcll(m));
+							gs = __shfl(gs, leader, WAVE) + (unsigned long long)__popcll(m & lanemask_lt());
 							if ((int64_t)gs < a.pcap)
 								a.pairs[gs] = p;
 						}
@@ -1206,6 +1212,22 @@ __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
 				s_nstage = 0;
 			__syncthreads();
 		}
+	};
+
+	// two register buffers in turn: the next bucket's records are in flight
+	// while the current one is binned and walked (no copies between them)
+	uint64_t kA[BK_PER], kB[BK_PER];
+	uint32_t sA, eA, sB, eB;
+	bucket_load(a, blockIdx.x, kA, &sA, &eA);
+	__syncthreads();
+	uint32_t iter = 0;
+	for (int b = blockIdx.x; b < a.nb; b += 2 * G) {  // uniform over the workgroup
+		bucket_load(a, b + G, kB, &sB, &eB);
+		process(b, iter++, kA, (int)(eA - sA));
+		if (b + G >= a.nb)
+			break;
+		bucket_load(a, b + 2 * G, kA, &sA, &eA);
+		process(b + G, iter++, kB, (int)(eB - sB));
 	}
 }
 
@@ -1223,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_big_counts(const Run* __restrict__ runs
 __global__ __launch_bounds__(256) void k_pairs_big(const uint64_t* __restrict__ rec,
                                                    const Run* __restrict__ runs,
                                                    const uint64_t* __restrict__ pre /* nruns+1 */,
-                                                   int64_t nruns, int idb,
+                                                   int64_t nruns, int idb, uint32_t lo, uint32_t hi,
                                                    uint64_t* __restrict__ pairs, int64_t cap,
                                                    RunCtrs* __restrict__ ctr) {
 	const uint64_t total = pre[nruns];
@@ -1249,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_pairs_big(const uint64_t* __restrict__ 
 				++b;
 			const uint64_t a = u - b * (b - 1) / 2;
 			const uint32_t st = runs[lo].start;
-			ok = pair_of(rec[st + a], rec[st + b], idb, &p);
+			ok = pair_of(rec[st + a], rec[st + b], idb, lo, hi, &p);
 		}
 		const unsigned long long slot = wave_append(&ctr->n_cand, ok ? 1u : 0u);
 		if (ok && (int64_t)slot < cap)
@@ -1478,6 +1500,25 @@ __device__ __forceinline__ int pair_distance(const uint8_t* a, int la, const uin
 	}
 }
 
+// pair_distance with the distance fixed at compile time, so that a kernel
+// holds only the band it uses: D in 1..8 is that distance's band (Hamming
+// needs none), D = 0 the wide band of a Levenshtein distance above 8
+template <int D>
+__device__ __forceinline__ int pair_distance_d(const uint8_t* a, int la, const uint8_t* b, int lb,
+                                               int distance, int measure) {
+	if constexpr (D == 0) {
+		return lev_band_rt(a, la, b, lb, distance);
+	} else {
+		if (la > REG_LEN || lb > REG_LEN)
+			return measure == 1 ? hamming(a, la, b, lb) : lev_band<D>(a, la, b, lb);
+		uint32_t A[REG_WORDS], B[REG_WORDS];
+		load_residues(a, la, A);
+		load_residues(b, lb, B);
+		if (measure == 1)
+			return hamming_reg(A, la, B, lb);
+		return lev_band_reg<D>(A, la, B, lb);
+	}
+}
 
 // ------------------------------------------- unique + verify + select --
 

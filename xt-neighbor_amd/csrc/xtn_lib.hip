@@ -26,6 +26,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <vector>
@@ -34,6 +35,7 @@
 #include "xtn_kernels.hpp"
 #include "xtn_overlap.hpp"
 #include "xtn_serve.hpp"
+#include "xtn_tail.hpp"
 
 using namespace xtn;
 
@@ -88,13 +90,32 @@ struct xtn_handle {
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAcc> prof;
 	std::vector<PendingEvent> spans;  // stage timing, always on
-	DBuf bstart, ovf, set_of, set_offs, tseq;
+	DBuf bstart, ovf, set_of, set_offs, tseq, tail_state;
 	int64_t last_pairs = 0;
 	int64_t last_unique = 0, last_out = 0;  // the previous run's unique candidates and outputs
 	                                         // (byte counts of the output select's profile)
 	// host copy of the prefix lengths, and per-row prefix sums of the
 	// variant counts for one distance: V of any row range without a readback
 	int n_cu = 256;
+	// the single-chunk run captured as a graph: replayed while a run's
+	// signature (sizes, pointers, the allocation generation) matches
+	struct GraphSig {
+		int64_t nseq, row_base, V, pcap, alloc_gen;
+		const void *seq_map, *set_of, *chars, *offs;
+		int32_t d, measure, bucket_cap, pad;
+		bool operator==(const GraphSig& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
+	};
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	GraphSig graph_sig{}, graph_want{};
+	bool graph_want_set = false;
+	int64_t graph_launches = 0;  // kernels the graph holds
+	bool capturing = false;
+	int64_t alloc_gen = 0;  // bumped by every device allocation of the handle
+	bool tail_safe = false;  // a look-back wait hit its limit once: host-checked tail from then on
+	uint32_t spin_limit = LB_SPIN_LIMIT;  // XTN_LOOKBACK_SPIN_LIMIT overrides (tests of the fallback)
+	size_t free_cache = 0;   // hipMemGetInfo's free bytes, re-read after an allocation
+	bool free_valid = false;
 	std::vector<uint8_t> prefix_len;
 	std::vector<uint64_t> vprefix;
 	int vprefix_d = 0;
@@ -123,10 +144,18 @@ int fail(xtn_handle* h, int code, const char* fmt, ...) {
 
 #define HIP_LAUNCH_CHECK(h)                                                                   \
 	do {                                                                                      \
+		++(h)->stats.n_launches;                                                              \
 		hipError_t _e = hipGetLastError();                                                    \
 		if (_e != hipSuccess)                                                                 \
 			return fail((h), XTN_ERR_HIP, "kernel launch failed: %s (%s:%d)",                 \
 			            hipGetErrorString(_e), __FILE__, __LINE__);                            \
+	} while (0)
+
+// a host wait on a stream, counted in the run's stats
+#define HIP_SYNC(h, stream)                            \
+	do {                                               \
+		++(h)->stats.n_syncs;                          \
+		HIP_TRY(h, hipStreamSynchronize(stream));      \
 	} while (0)
 
 #define TRY(expr)                 \
@@ -141,8 +170,10 @@ int ensure(xtn_handle* h, DBuf& b, size_t bytes) {
 		bytes = 256;
 	if (bytes <= b.cap)
 		return XTN_OK;
+	if (h && h->capturing)  // the caller abandons the capture and issues the run eagerly
+		return fail(h, XTN_ERR_STATE, "allocation during graph capture");
 	if (b.p) {
-		HIP_TRY(h, hipStreamSynchronize(h->stream));
+		HIP_SYNC(h, h->stream);
 		HIP_TRY(h, hipFree(b.p));
 		b.p = nullptr;
 		b.cap = 0;
@@ -161,6 +192,10 @@ int ensure(xtn_handle* h, DBuf& b, size_t bytes) {
 		}
 	}
 	b.cap = want;
+	if (h) {
+		h->free_valid = false;
+		++h->alloc_gen;
+	}
 	return XTN_OK;
 }
 
@@ -179,7 +214,7 @@ int grow_keep(xtn_handle* h, DBuf& b, size_t bytes, size_t keep) {
 	TRY(ensure(h, nb, bytes));
 	if (keep)
 		HIP_TRY(h, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, h->stream));
-	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	HIP_SYNC(h, h->stream);
 	release(b);
 	b = nb;
 	return XTN_OK;
@@ -208,6 +243,7 @@ inline unsigned grid_for(int64_t n, int per_block) {
 static int tile_seq_table(xtn_handle* h, hipStream_t s, int32_t n, uint64_t V, const uint32_t** out) {
 	const uint64_t ntiles = (V + GEN_TILE - 1) / GEN_TILE;
 	TRY(ensure(h, h->tseq, sizeof(uint32_t) * (size_t)(ntiles + 1)));
+	*out = h->tseq.as<uint32_t>();
 	k_tile_seq<<<grid_for(n, 256), 256, 0, s>>>(h->voff.as<uint64_t>(), n, V, (uint32_t)ntiles,
 	                                            h->tseq.as<uint32_t>());
 	HIP_LAUNCH_CHECK(h);
@@ -269,12 +305,16 @@ void collect_profile(xtn_handle* h) {
 	h->pending.clear();
 }
 
-// stage spans: a pair of events per stage and chunk, summed after the run
+// stage spans: a pair of events per stage and chunk, summed after the run;
+// recorded only in verbose or profile mode (an event record between two
+// kernels leaves the GPU idle for several microseconds)
 struct StageScope {
 	xtn_handle* h;
 	PendingEvent pe;
 	bool on = false;
 	StageScope(xtn_handle* hh, const char* stage) : h(hh) {
+		if (!hh->cfg.verbose && !hh->cfg.profile)
+			return;
 		pe.name = stage;
 		pe.bytes = 0;
 		pe.start = take_event(h);
@@ -419,7 +459,7 @@ int radix_sort(xtn_handle* h, DBuf& keys, DBuf& keys_alt, DBuf* vals, DBuf* vals
 
 int read_ctrs(xtn_handle* h, RunCtrs* out) {
 	HIP_TRY(h, hipMemcpyAsync(out, h->ctrs.p, sizeof(RunCtrs), hipMemcpyDeviceToHost, h->stream));
-	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	HIP_SYNC(h, h->stream);
 	return XTN_OK;
 }
 
@@ -440,6 +480,24 @@ double now_ms() {
 constexpr int64_t BUCKET_TARGET = 2200;  // mean records per LDS bucket (<= BK_CAP fit)
 constexpr int KEY_BITS = 32;             // hash bits that order records into runs
 
+// device control block: the counters of the candidate search, then the
+// tail's block
+struct CtlBlock {
+	RunCtrs rc;
+	TailCtl tc;
+};
+// what a run reads back: the search counters and the tail's scalars
+struct CtlHead {
+	RunCtrs rc;
+	uint32_t error, pad0;
+	unsigned long long n_unique, n_out;
+};
+static_assert(offsetof(CtlBlock, tc) == sizeof(RunCtrs), "the tail block follows the counters");
+static_assert(sizeof(CtlHead) == sizeof(RunCtrs) + offsetof(TailCtl, pad1), "head of the control block");
+static_assert(sizeof(CtlBlock) % 8 == 0 && sizeof(RunCtrs) % 8 == 0, "filled in u64 words");
+
+inline CtlBlock* ctl_block(xtn_handle* h) { return h->ctrs.as<CtlBlock>(); }
+
 inline unsigned long long* ctr_field(xtn_handle* h, size_t offset) {
 	return reinterpret_cast<unsigned long long*>(static_cast<char*>(h->ctrs.p) + offset);
 }
@@ -450,10 +508,29 @@ int set_ctr(xtn_handle* h, size_t offset, unsigned long long v) {
 	return XTN_OK;
 }
 
+// zeroes the search counters and/or the tail block with one fill launch (a
+// kernel, not a memset, so that the sequence can be captured in a graph)
+int zero_ctl(xtn_handle* h, bool counters, bool tail) {
+	const size_t b0 = counters ? 0 : sizeof(RunCtrs);
+	const size_t b1 = tail ? sizeof(CtlBlock) : sizeof(RunCtrs);
+	const int64_t words = (int64_t)(b1 - b0) / 8;
+	k_fill_u64<<<grid_for(words, 256), 256, 0, h->stream>>>(ctr_field(h, b0), words, 0ull);
+	HIP_LAUNCH_CHECK(h);
+	return XTN_OK;
+}
+
+int read_head(xtn_handle* h, CtlHead* out) {
+	HIP_TRY(h, hipMemcpyAsync(out, h->ctrs.p, sizeof(CtlHead), hipMemcpyDeviceToHost, h->stream));
+	HIP_SYNC(h, h->stream);
+	return XTN_OK;
+}
+
 // Global path for one bucket that does not fit the LDS tile: sort the
-// segment by the run bits, list its runs, emit the pairs of every run.
+// segment by the run bits, list its runs, emit the pairs of every run whose
+// smaller id lies in [lo, hi). *split is set (and nothing emitted) when the
+// segment's pairs could exceed pmax candidates in all.
 int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, int sort_hi, int idb,
-                    int64_t* n_pairs) {
+                    uint32_t lo, uint32_t hi, int64_t pmax, int64_t* n_pairs, bool* split) {
 	hipStream_t s = h->stream;
 	uint64_t* seg = h->recs.as<uint64_t>() + seg_start;
 	uint64_t* k = seg;
@@ -478,8 +555,10 @@ int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, i
 		return fail(h, XTN_ERR_HIP, "run list overflow (%llu small, %llu big)",
 		            (unsigned long long)rc.n_small, (unsigned long long)rc.n_big);
 	const uint64_t bound = rc.pairs_small + rc.pairs_big;
-	if ((uint64_t)*n_pairs + bound > (uint64_t)INT32_MAX)
-		return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31");
+	if ((uint64_t)*n_pairs + bound > (uint64_t)pmax) {
+		*split = true;
+		return XTN_OK;
+	}
 	const int64_t need = *n_pairs + (int64_t)bound;
 	if ((size_t)need * sizeof(uint64_t) > h->pairs.cap)
 		TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)need, sizeof(uint64_t) * (size_t)*n_pairs));
@@ -487,7 +566,7 @@ int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, i
 	if (rc.n_small) {
 		ProfScope ps(h, "pairs_small", (double)rc.n_small * 8.0);
 		k_pairs_small<<<grid_for((int64_t)rc.n_small, 256), 256, 0, s>>>(
-		    seg, h->runs_small.as<Run>(), (int64_t)rc.n_small, idb, h->pairs.as<uint64_t>(), pcap,
+		    seg, h->runs_small.as<Run>(), (int64_t)rc.n_small, idb, lo, hi, h->pairs.as<uint64_t>(), pcap,
 		    h->ctrs.as<RunCtrs>());
 		HIP_LAUNCH_CHECK(h);
 	}
@@ -501,7 +580,7 @@ int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, i
 		const unsigned g = (unsigned)std::min<uint64_t>((rc.pairs_big + 255) / 256, 16384);
 		ProfScope ps(h, "pairs_big", (double)rc.pairs_big * 16.0);
 		k_pairs_big<<<g > 0 ? g : 1, 256, 0, s>>>(seg, h->runs_big.as<Run>(), h->big_pre.as<uint64_t>(),
-		                                          (int64_t)rc.n_big, idb, h->pairs.as<uint64_t>(), pcap,
+		                                          (int64_t)rc.n_big, idb, lo, hi, h->pairs.as<uint64_t>(), pcap,
 		                                          h->ctrs.as<RunCtrs>());
 		HIP_LAUNCH_CHECK(h);
 	}
@@ -514,20 +593,313 @@ int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, i
 	return XTN_OK;
 }
 
+// ---- the candidate tail (xtn_tail.hpp): sort, unique, verify, select --
+
+struct TailPlan {
+	int64_t pcap = 0;  // candidates the launches are sized for
+	int passes = 0;    // 8-bit LSD digits over the 2 * idb pair bits
+	int64_t sort_tiles = 0, sel_tiles = 0;
+};
+
+// buffers of a tail over up to pcap candidates whose output goes after the
+// out_keep pairs already held: the sort's second buffer, the outputs and the
+// look-back state
+int tail_prepare(xtn_handle* h, int64_t pcap, int idb, int64_t out_keep, TailPlan* tp) {
+	tp->pcap = pcap;
+	tp->passes = (2 * idb + 7) / 8;
+	tp->sort_tiles = (pcap + RS_TILE - 1) / RS_TILE;
+	tp->sel_tiles = (pcap + SL_TILE - 1) / SL_TILE;
+	TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)pcap));
+	TRY(grow_keep(h, h->out_pairs, sizeof(int2) * (size_t)(out_keep + pcap), sizeof(int2) * (size_t)out_keep));
+	TRY(grow_keep(h, h->out_dist, (size_t)(out_keep + pcap), (size_t)out_keep));
+	// the sort's 32-bit words, then the select's 64-bit words (the sort part
+	// is a multiple of RS_BINS words, so the select's start is 8-aligned)
+	TRY(ensure(h, h->tail_state,
+	           sizeof(uint32_t) * (size_t)((int64_t)tp->passes * tp->sort_tiles * RS_BINS + 2 * tp->sel_tiles)));
+	return XTN_OK;
+}
+
+// the tail's launches over the candidates in `pairs` (count: the search
+// counter), outputs written from out_pairs[out_off]; the tail block must be
+// zero (zero_ctl)
+int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_map, uint32_t row_base,
+               const uint32_t* set_of, int64_t out_off) {
+	hipStream_t s = h->stream;
+	CtlBlock* cb = ctl_block(h);
+	const unsigned long long* n_dev = &cb->rc.n_cand;
+	uint32_t* state = h->tail_state.as<uint32_t>();
+	const int64_t nsort = (int64_t)tp.passes * tp.sort_tiles * RS_BINS;
+	// profile byte counts: the candidates of the previous run (the count of
+	// this one is only on the device)
+	const double cand = (double)h->last_pairs;
+	{
+		ProfScope ps(h, "tail_hist", 8.0 * cand);
+		const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tp.sort_tiles, 256));
+		k_tail_hist<<<g, TH_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_dev, tp.pcap, tp.passes, &cb->tc, state,
+		                                nsort + 2 * tp.sel_tiles);
+		HIP_LAUNCH_CHECK(h);
+	}
+	uint64_t* a = h->pairs.as<uint64_t>();
+	uint64_t* b = h->pairs_alt.as<uint64_t>();
+	const unsigned sort_grid = (unsigned)std::max<int64_t>(tp.sort_tiles, 1);
+	for (int p = 0; p < tp.passes; ++p) {
+		ProfScope ps(h, "sort_pairs", 16.0 * cand);
+		k_onesweep<OS_NT><<<sort_grid, OS_NT, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc,
+		                                             state + (int64_t)p * tp.sort_tiles * RS_BINS, h->spin_limit);
+		HIP_LAUNCH_CHECK(h);
+		std::swap(a, b);
+	}
+	SelectArgs sa{};
+	sa.keys = a;
+	sa.n_dev = n_dev;
+	sa.cap = tp.pcap;
+	sa.idb = idb;
+	sa.distance = h->cfg.distance;
+	sa.measure = h->cfg.measure;
+	sa.row_base = row_base;
+	sa.chars = h->chars.as<uint8_t>();
+	sa.offs = h->offs.as<uint32_t>();
+	sa.seq_map = seq_map;
+	sa.set_of = set_of;
+	sa.out_pairs = h->out_pairs.as<int2>() + out_off;
+	sa.out_dist = h->out_dist.as<int8_t>() + out_off;
+	sa.ctl = &cb->tc;
+	sa.state = reinterpret_cast<uint64_t*>(state + nsort);
+	sa.spin_limit = h->spin_limit;
+	{
+		// candidates read once, the unique ones' two strings and offsets, the
+		// kept triples written
+		ProfScope ps(h, "select", 8.0 * cand + 40.0 * (double)h->last_unique + 9.0 * (double)h->last_out);
+		const unsigned g = (unsigned)std::max<int64_t>(1, tp.sel_tiles);
+		select_kernel(h->cfg.distance, h->cfg.measure)<<<g, SL_NT, 0, s>>>(sa);
+		HIP_LAUNCH_CHECK(h);
+	}
+	return XTN_OK;
+}
+
+// The same tail with the host in the loop, for a run whose look-back waits
+// hit their limit: the candidate count read back, an LSD sort of 8-bit
+// reduce-then-scan passes, then unique, verification and the order-preserving
+// select as separate launches. Fills head->n_unique and head->n_out.
+int tail_rts(xtn_handle* h, int64_t n_pairs, int idb, const uint32_t* seq_map, uint32_t row_base,
+             const uint32_t* set_of, int64_t out_off, CtlHead* head) {
+	hipStream_t s = h->stream;
+	head->n_unique = head->n_out = 0;
+	if (n_pairs <= 0)
+		return XTN_OK;
+	TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)n_pairs));
+	TRY(radix_sort(h, h->pairs, h->pairs_alt, nullptr, nullptr, n_pairs, 0, 2 * idb, "sort_pairs_rts"));
+	const int64_t nt = (n_pairs + SEL_TILE - 1) / SEL_TILE;  // unique step tiles
+	const int64_t nv = (n_pairs + SEL_NT - 1) / SEL_NT;      // verify step tiles
+	TRY(ensure(h, h->pos, sizeof(uint32_t) * (size_t)(nt + 1 + nv + 1)));
+	TRY(ensure(h, h->dist, (size_t)n_pairs));
+	TRY(grow_keep(h, h->out_pairs, sizeof(int2) * (size_t)(out_off + n_pairs), sizeof(int2) * (size_t)out_off));
+	TRY(grow_keep(h, h->out_dist, (size_t)(out_off + n_pairs), (size_t)out_off));
+	uint32_t* ucnt = h->pos.as<uint32_t>();  // nt counts, total at [nt]
+	uint32_t* kcnt = ucnt + nt + 1;          // nv counts of kept pairs, total at [nv]
+	uint64_t* upairs = h->pairs_alt.as<uint64_t>();
+	k_uniq_count<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt);
+	HIP_LAUNCH_CHECK(h);
+	k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(ucnt, nt, ucnt + nt, nullptr, 1);
+	HIP_LAUNCH_CHECK(h);
+	k_uniq_write<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt, upairs);
+	HIP_LAUNCH_CHECK(h);
+	auto verify = h->cfg.measure == 0 && h->cfg.distance > 8 ? k_verify_count<true> : k_verify_count<false>;
+	verify<<<(unsigned)nv, SEL_NT, 0, s>>>(upairs, ucnt + nt, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
+	                                       seq_map, row_base, set_of, h->cfg.distance, h->cfg.measure,
+	                                       h->dist.as<int8_t>(), kcnt);
+	HIP_LAUNCH_CHECK(h);
+	k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(kcnt, nv, kcnt + nv, ucnt + nt, SEL_NT);
+	HIP_LAUNCH_CHECK(h);
+	k_keep_write<<<(unsigned)nv, SEL_NT, 0, s>>>(upairs, ucnt + nt, idb, h->dist.as<int8_t>(), kcnt,
+	                                             h->out_pairs.as<int2>() + out_off, h->out_dist.as<int8_t>() + out_off);
+	HIP_LAUNCH_CHECK(h);
+	uint32_t totals[2] = {0, 0};
+	HIP_TRY(h, hipMemcpyAsync(&totals[0], ucnt + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_TRY(h, hipMemcpyAsync(&totals[1], kcnt + nv, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+	HIP_SYNC(h, s);
+	head->n_unique = totals[0];
+	head->n_out = totals[1];
+	return XTN_OK;
+}
+
+// ---- grouping of one key-space chunk's records into buckets --
+
+struct Grouping {
+	int64_t nrec = 0;
+	int64_t nseq = 0;  // ids of the run: a round over [0, nseq) is not ranged
+	int B = 0, SB = 0, nb = 1;
+	int hi = 64, rs = 0;  // the chunk's hash bits end at hi; runs are equal rec >> rs
+	bool region = false;  // fixed bucket regions (more than 8 bucket bits)
+	bool fused_hist = false;
+	int64_t rtiles = 0;
+	BucketArgs ba{};
+};
+
+// records in recs (written by the generator) -> the bucket layout ba names:
+// region layout (exact pass over the top 8 bucket bits into recs_alt, then
+// the region scatter back into recs) or, with at most 8 bucket bits, the
+// sorted layout with bucket bounds
+int issue_grouping(xtn_handle* h, Grouping& g) {
+	hipStream_t s = h->stream;
+	BucketArgs& ba = g.ba;
+	if (g.region) {
+		StageScope sg(h, "sort");
+		uint32_t* hist = h->hist.as<uint32_t>();
+		if (!g.fused_hist) {
+			ProfScope ps(h, "sort_rec_hist", (double)g.nrec * 8.0);
+			k_radix_hist<<<(unsigned)g.rtiles, RS_NT, 0, s>>>(h->recs.as<uint64_t>(), g.nrec, g.hi - 8, hist, g.rtiles);
+			HIP_LAUNCH_CHECK(h);
+		}
+		uint32_t* dtot = nullptr;
+		TRY(digit_scan(h, hist, g.rtiles, &dtot));
+		k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
+		HIP_LAUNCH_CHECK(h);
+		{
+			ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)g.nrec);
+			k_radix_scatter<false, RS_NT, true><<<(unsigned)g.rtiles, RS_NT, 0, s>>>(
+			    h->recs.as<uint64_t>(), nullptr, h->recs_alt.as<uint64_t>(), nullptr, g.nrec, g.hi - 8, hist, g.rtiles);
+			HIP_LAUNCH_CHECK(h);
+		}
+		k_fill_u64<<<grid_for(g.nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), g.nb / 2, 0ull);
+		HIP_LAUNCH_CHECK(h);
+		RegionArgs ra{};
+		ra.in = h->recs_alt.as<uint64_t>();
+		ra.out = h->recs.as<uint64_t>();
+		ra.ctl = h->ctl.as<uint32_t>();
+		ra.count = h->bcount.as<uint32_t>();
+		ra.stride = ba.cap;
+		ra.shift = g.hi - g.B;
+		ra.fine_bits = g.B - 8;
+		{
+			ProfScope ps(h, "region", 16.0 * (double)g.nrec);
+			k_region_scatter<<<(unsigned)(g.rtiles + RS_BINS), RS_NT, 0, s>>>(ra);
+			HIP_LAUNCH_CHECK(h);
+		}
+		ba.rec = h->recs.as<uint64_t>();
+		ba.bcount = h->bcount.as<uint32_t>();
+		ba.bstride = ba.cap;
+		ba.bstart = nullptr;
+		return XTN_OK;
+	}
+	if (g.B > 0) {
+		StageScope sg(h, "sort");
+		TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, g.nrec, g.hi - g.SB, g.hi, "sort_rec",
+		               g.fused_hist));
+	}
+	ba.rec = h->recs.as<uint64_t>();
+	ba.bcount = nullptr;
+	ba.bstart = h->bstart.as<uint32_t>();
+	k_bucket_bounds<<<grid_for(g.nb + 1, 256), 256, 0, s>>>(ba.rec, g.nrec, g.B > 0 ? g.hi - g.B : 0,
+	                                                        (uint32_t)(g.nb - 1), g.nb, h->bstart.as<uint32_t>());
+	HIP_LAUNCH_CHECK(h);
+	return XTN_OK;
+}
+
+// a bucket outgrew its region: the chunk's records, ordered by the top 8
+// bucket bits in recs_alt by the first pass, are sorted exactly by every
+// bucket bit and bucketed by bounds
+int regroup_exact(xtn_handle* h, Grouping& g) {
+	std::swap(h->recs, h->recs_alt);
+	g.region = false;
+	g.fused_hist = false;
+	return issue_grouping(h, g);
+}
+
+int issue_buckets(xtn_handle* h, Grouping& g, uint32_t lo, uint32_t hi, int64_t pcap) {
+	BucketArgs& ba = g.ba;
+	ba.pairs = h->pairs.as<uint64_t>();
+	ba.pcap = pcap;
+	ba.id_lo = lo;
+	ba.id_hi = hi;
+	const bool ranged = lo > 0 || (int64_t)hi < g.nseq;
+	StageScope sg(h, "pairs");
+	ProfScope ps(h, "bucket", (double)g.nrec * 8.0);
+	// persistent workgroups, three per CU
+	const unsigned grid = (unsigned)std::min<int64_t>(g.nb, (int64_t)h->n_cu * 3);
+	if (ranged)
+		k_bucket<true><<<grid, BK_NT, 0, h->stream>>>(ba);
+	else
+		k_bucket<false><<<grid, BK_NT, 0, h->stream>>>(ba);
+	HIP_LAUNCH_CHECK(h);
+	return XTN_OK;
+}
+
+// The candidates of one chunk with the host checking each step: the bucket
+// kernel (candidate counter starting at *n_pairs), the pair buffer regrown
+// when it overflowed (keeping the *n_pairs already held), a region layout
+// whose bucket overflowed regrouped exactly, buckets too large for LDS
+// through the global path. *split is set when the round's candidates would
+// exceed pmax; *pcap follows the buffer.
+int checked_candidates(xtn_handle* h, Grouping& g, uint32_t lo, uint32_t hi, int64_t pmax, int64_t* pcap,
+                       int64_t* n_pairs, bool* split) {
+	hipStream_t s = h->stream;
+	const int64_t n0 = *n_pairs;
+	for (int attempt = 0; attempt < 6; ++attempt) {
+		TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
+		TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)n0));
+		TRY(issue_buckets(h, g, lo, hi, *pcap));
+		RunCtrs rc{};
+		TRY(read_ctrs(h, &rc));
+		if ((int64_t)rc.n_cand > *pcap) {
+			const int64_t need = (int64_t)rc.n_cand + (int64_t)rc.n_cand / 4;
+			if ((int64_t)rc.n_cand > pmax) {
+				*split = true;
+				return XTN_OK;
+			}
+			const int64_t grow = std::min(need, pmax);
+			TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)grow, sizeof(uint64_t) * (size_t)n0));
+			*pcap = grow;
+			continue;
+		}
+		if (rc.append > 0 && g.region) {
+			TRY(regroup_exact(h, g));
+			continue;
+		}
+		int64_t np = (int64_t)rc.n_cand;
+		if (rc.append > 0) {
+			StageScope sg(h, "pairs");
+			const int64_t n_ovf = (int64_t)rc.append;
+			std::vector<uint32_t> ids((size_t)n_ovf), bs((size_t)g.nb + 1);
+			HIP_TRY(h, hipMemcpyAsync(ids.data(), h->ovf.p, sizeof(uint32_t) * (size_t)n_ovf, hipMemcpyDeviceToHost, s));
+			HIP_TRY(h, hipMemcpyAsync(bs.data(), h->bstart.p, sizeof(uint32_t) * ((size_t)g.nb + 1),
+			                          hipMemcpyDeviceToHost, s));
+			HIP_SYNC(h, s);
+			for (uint32_t b : ids) {
+				TRY(bucket_fallback(h, bs[b], (int64_t)bs[b + 1] - bs[b], g.rs, g.hi - g.B, g.ba.idb, lo, hi, pmax,
+				                    &np, split));
+				if (*split)
+					return XTN_OK;
+			}
+			*pcap = std::max<int64_t>(*pcap, np);
+		}
+		*n_pairs = np;
+		return XTN_OK;
+	}
+	return fail(h, XTN_ERR_HIP, "candidate search did not settle");
+}
+
 // Runs the candidate search and verification over `nseq` ids. seq_map (device,
 // nullable) maps an id to its input row. Leaves sorted (x, y, distance)
 // triples in out_pairs/out_dist and their count in h->out_len.
+//
+// One key-space chunk (the usual case on a 288 GB device): generation,
+// grouping, the bucket kernel and the whole tail are issued back to back and
+// the host reads the counters once at the end. Only a pair buffer that was
+// too small or a bucket too large for LDS (both seen in those counters) send
+// the round through checked_candidates. Candidates beyond the round budget
+// (pmax) split the smaller-id range into rounds whose outputs follow each
+// other in order (the reference's lower-bound rounds, xtn.cu:385-470).
 int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t row_base = 0,
                 const uint32_t* set_of = nullptr) {
 	hipStream_t s = h->stream;
 	const int d = h->cfg.distance;
-	const int measure = h->cfg.measure;
 	xtn_stats& st = h->stats;
 	const double t0 = now_ms();
 	h->out_len = 0;
 	st.n_sequences = nseq;
 	if (nseq < 2) {
-		HIP_TRY(h, hipStreamSynchronize(s));
+		HIP_SYNC(h, s);
 		st.ms_total = now_ms() - t0;
 		return XTN_OK;
 	}
@@ -538,20 +910,10 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	// pack + variant offsets
 	TRY(ensure(h, h->pk, sizeof(uint64_t) * 2 * (size_t)nseq));
 	TRY(ensure(h, h->voff, sizeof(uint64_t) * ((size_t)nseq + 1)));
-	TRY(ensure(h, h->ctrs, sizeof(RunCtrs)));
-	{
-		StageScope sg(h, "generate");
-		ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
-		k_pack<<<grid_for(nseq, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(),
-		                                            seq_map, row_base, set_of, nseq, d, h->pk.as<uint64_t>(),
-		                                            h->voff.as<uint64_t>());
-		HIP_LAUNCH_CHECK(h);
-	}
-	TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), nseq,
-	                        h->voff.as<uint64_t>() + nseq));
+	TRY(ensure(h, h->ctrs, sizeof(CtlBlock)));
 	uint64_t V = 0;
 	if (!seq_map && (int64_t)row_base + nseq <= (int64_t)h->prefix_len.size()) {
-		// the same closed form the device scan just summed
+		// the closed form the device scan sums
 		if (h->vprefix_d != d || h->vprefix.size() != h->prefix_len.size() + 1) {
 			uint64_t per[19];
 			for (int L = 0; L <= 18; ++L)
@@ -562,21 +924,40 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			h->vprefix_d = d;
 		}
 		V = h->vprefix[row_base + nseq] - h->vprefix[row_base];
-	} else {
-		HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + nseq, sizeof(uint64_t),
-		                          hipMemcpyDeviceToHost, s));
-		HIP_TRY(h, hipStreamSynchronize(s));
+	}
+	auto issue_pack = [&]() -> int {
+		StageScope sg(h, "generate");
+		{
+			ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
+			k_pack<<<grid_for(nseq, 256), 256, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), seq_map,
+			                                            row_base, set_of, nseq, d, h->pk.as<uint64_t>(),
+			                                            h->voff.as<uint64_t>());
+			HIP_LAUNCH_CHECK(h);
+		}
+		TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), nseq,
+		                        h->voff.as<uint64_t>() + nseq));
+		return XTN_OK;
+	};
+	const bool v_known = V > 0;
+	if (!v_known) {
+		TRY(issue_pack());
+		HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + nseq, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+		HIP_SYNC(h, s);
 	}
 	st.n_variants = (int64_t)V;
 
 	// chunk plan: the records of one chunk (plus the sort buffer and the
 	// bucket regions, ~40 B per record in all) must fit the HBM budget, and
 	// its buckets must stay near BUCKET_TARGET records
+	if (!h->free_valid) {
+		size_t total_b = 0;
+		HIP_TRY(h, hipMemGetInfo(&h->free_cache, &total_b));
+		h->free_valid = true;
+	}
+	const size_t free_b = h->free_cache;
 	int64_t budget = h->cfg.max_records_per_chunk;
 	if (budget <= 0) {
-		size_t fr = 0, tot = 0;
-		HIP_TRY(h, hipMemGetInfo(&fr, &tot));
-		budget = (int64_t)((double)fr * 0.6 / 40.0);
+		budget = (int64_t)((double)free_b * 0.6 / 40.0);
 		budget = std::min<int64_t>(budget, (int64_t)65536 * BUCKET_TARGET);
 	}
 	budget = std::max<int64_t>(budget, 1 << 16);
@@ -595,6 +976,12 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	const int rs = hi - K;
 	st.n_chunks = nparts;
 
+	// candidates per round: the look-back counts (30 bits) and ~30 bytes of
+	// HBM per candidate (two sort buffers, the outputs, the state)
+	int64_t pmax = h->cfg.max_pairs_per_round > 0 ? h->cfg.max_pairs_per_round
+	                                               : (int64_t)((double)free_b * 0.3 / 30.0);
+	pmax = std::max<int64_t>(std::min<int64_t>(pmax, (int64_t)LB_VAL), 1024);
+
 	GenArgs ga{};
 	ga.pk = h->pk.as<uint64_t>();
 	ga.voff = h->voff.as<uint64_t>();
@@ -604,13 +991,23 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	ga.part_bits = pbits;
 	ga.V = (int64_t)V;
 	ga.set_of = set_of;
-	if (V > 0)
+	// packing (unless done above to learn V) and the generator's tile table
+	bool prologue_done = false;
+	auto prologue = [&]() -> int {
+		if (prologue_done)
+			return XTN_OK;
+		if (v_known)
+			TRY(issue_pack());
 		TRY(tile_seq_table(h, s, nseq, V, &ga.tile_seq));
+		prologue_done = true;
+		return XTN_OK;
+	};
 	const unsigned gen_grid = grid_for((int64_t)V, GEN_TILE);
 
 	std::vector<unsigned long long> part_count(nparts, 0);
 	int64_t cap_recs = (int64_t)V;
 	if (nparts > 1) {
+		TRY(prologue());
 		TRY(ensure(h, h->part_hist, sizeof(unsigned long long) * nparts));
 		k_fill_u64<<<grid_for(nparts, 256), 256, 0, s>>>(h->part_hist.as<unsigned long long>(), nparts, 0ull);
 		ga.part_hist = h->part_hist.as<unsigned long long>();
@@ -622,7 +1019,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		}
 		HIP_TRY(h, hipMemcpyAsync(part_count.data(), h->part_hist.p, sizeof(unsigned long long) * nparts,
 		                          hipMemcpyDeviceToHost, s));
-		HIP_TRY(h, hipStreamSynchronize(s));
+		HIP_SYNC(h, s);
 		cap_recs = 0;
 		for (auto c : part_count)
 			cap_recs = std::max<int64_t>(cap_recs, (int64_t)c);
@@ -641,242 +1038,268 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	TRY(ensure(h, h->ovf, sizeof(uint32_t) * 65536));
 	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, BK_CAP) : BK_CAP;
 
-	// candidate pair buffer, sized from the last run; regrown on overflow
-	const int64_t pguess = std::max<int64_t>({h->last_pairs + h->last_pairs / 4, (int64_t)V / 16, 1 << 16});
-	if ((size_t)pguess * sizeof(uint64_t) > h->pairs.cap)
-		TRY(ensure(h, h->pairs, sizeof(uint64_t) * (size_t)pguess));
-
-	int64_t n_pairs = 0;
-	for (int part = 0; part < nparts; ++part) {
-		const int64_t nrec = (int64_t)part_count[part];
-		if (nrec < 2)
-			continue;
-		ga.out = h->recs.as<uint64_t>();
-		ga.cap = cap_recs;
+	// the grouping plan of a chunk of nrec records
+	auto plan_chunk = [&](int64_t nrec, Grouping& g) -> int {
+		g = Grouping{};
+		g.nrec = nrec;
+		g.nseq = nseq;
 		// bucket bits: the B hash bits right below the chunk bits, B such that
 		// buckets hold about BUCKET_TARGET records; the sort orders the
 		// records by the next multiple of 8 bits
 		int B = 0;
 		while (B < 16 && (nrec >> B) > BUCKET_TARGET)
 			++B;
-		B = std::min(B, K);
-		const int SB = (B + 7) / 8 * 8;
+		g.B = std::min(B, K);
+		g.SB = (g.B + 7) / 8 * 8;
+		g.nb = 1 << g.B;
+		g.hi = hi;
+		g.rs = rs;
 		// more than 8 bucket bits: an exact pass over the top 8, then the
 		// region scatter (fixed regions of bucket_cap slots in recs)
-		const bool region = B > 8;
-		if (region) {
-			TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)((int64_t)bucket_cap << B)));
-			ga.out = h->recs.as<uint64_t>();
-		}
+		g.region = g.B > 8;
+		TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)cap_recs));
+		TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)cap_recs));
+		if (g.region)
+			TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)((int64_t)bucket_cap << g.B)));
 		// the single-chunk generator also counts the first sort digit per tile
-		const bool fused_hist = nparts == 1 && B > 0;
-		const int64_t rtiles = (nrec + RS_TILE - 1) / RS_TILE;
-		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * rtiles)));
-		if (fused_hist) {
-			ga.digit_hist = h->hist.as<uint32_t>();
-			ga.ntiles = rtiles;
-			ga.digit_shift = region ? hi - 8 : hi - SB;
-		} else {
-			ga.digit_hist = nullptr;
-		}
-		{
-			StageScope sg(h, "generate");
-			if (nparts == 1) {
-				ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (fused_hist ? 1024.0 * rtiles : 0.0));
-				k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
-				HIP_LAUNCH_CHECK(h);
-			} else {
-				ga.part_id = (uint32_t)part;
-				ga.append = ctr_field(h, offsetof(RunCtrs, append));
-				TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
-				ProfScope ps(h, "gen", (double)nrec * 8.0 + (double)nseq * 24.0);
-				k_gen<GEN_APPEND><<<gen_grid, GEN_NT, 0, s>>>(ga);
-				HIP_LAUNCH_CHECK(h);
-			}
-		}
-		const int nb = 1 << B;
-		BucketArgs ba{};
-		ba.nb = nb;
-		// bins: the (up to) 11 stored hash bits right below the bucket bits
-		ba.bin_bits = std::min(BK_BIN_BITS, hi - B - idb);
-		ba.bin_shift = hi - B - ba.bin_bits;
+		g.fused_hist = nparts == 1 && g.B > 0;
+		g.rtiles = (nrec + RS_TILE - 1) / RS_TILE;
+		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * g.rtiles)));
+		BucketArgs& ba = g.ba;
+		ba.nb = g.nb;
+		// bins: the (up to) 12 stored hash bits right below the bucket bits
+		ba.bin_bits = std::min(BK_BIN_BITS, hi - g.B - idb);
+		ba.bin_shift = hi - g.B - ba.bin_bits;
 		ba.idb = idb;
 		ba.cap = bucket_cap;
 		ba.ovf_cap = 65536;
 		ba.ctr = h->ctrs.as<RunCtrs>();
 		ba.ovf = h->ovf.as<uint32_t>();
 		ba.n_ovf = ctr_field(h, offsetof(RunCtrs, append));
-		const int64_t n_pairs_chunk0 = n_pairs;
-		RunCtrs rc{};
-		// bucket kernel over the layout ba names, the pair buffer regrown once
-		// if the candidates did not fit
-		auto run_buckets = [&]() -> int {
-			for (int attempt = 0; attempt < 2; ++attempt) {
-				StageScope sg(h, "pairs");
-				if (!ba.bcount) {
-					k_bucket_bounds<<<grid_for(nb + 1, 256), 256, 0, s>>>(
-					    ba.rec, nrec, B > 0 ? hi - B : 0, (uint32_t)(nb - 1), nb, h->bstart.as<uint32_t>());
-					HIP_LAUNCH_CHECK(h);
-				}
-				TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
-				TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)n_pairs_chunk0));
-				ba.pairs = h->pairs.as<uint64_t>();
-				ba.pcap = (int64_t)(h->pairs.cap / sizeof(uint64_t));
-				{
-					ProfScope ps(h, "bucket", (double)nrec * 8.0);
-					// persistent workgroups, three per CU
-					const unsigned grid = (unsigned)std::min<int64_t>(nb, (int64_t)h->n_cu * 3);
-					k_bucket<<<grid, BK_NT, 0, s>>>(ba);
-					HIP_LAUNCH_CHECK(h);
-				}
-				TRY(read_ctrs(h, &rc));
-				if ((int64_t)rc.n_cand <= ba.pcap)
-					return XTN_OK;
-				if (attempt == 1)
-					return fail(h, XTN_ERR_HIP, "pair buffer overflow after regrow");
-				if (rc.n_cand > (unsigned long long)INT32_MAX)
-					return fail(h, XTN_ERR_CAPACITY, "candidate pairs exceed 2^31 (%llu)",
-					            (unsigned long long)rc.n_cand);
-				TRY(grow_keep(h, h->pairs, sizeof(uint64_t) * (size_t)(rc.n_cand + rc.n_cand / 4),
-				              sizeof(uint64_t) * (size_t)n_pairs_chunk0));
-			}
-			return XTN_OK;
-		};
+		return XTN_OK;
+	};
+	// the generator launch of one chunk (records into recs)
+	auto issue_gen = [&](int part, const Grouping& g) -> int {
+		ga.out = h->recs.as<uint64_t>();
+		ga.cap = cap_recs;
+		if (g.fused_hist) {
+			ga.digit_hist = h->hist.as<uint32_t>();
+			ga.ntiles = g.rtiles;
+			ga.digit_shift = g.region ? hi - 8 : hi - g.SB;
+		} else {
+			ga.digit_hist = nullptr;
+		}
+		StageScope sg(h, "generate");
+		if (nparts == 1) {
+			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (g.fused_hist ? 1024.0 * g.rtiles : 0.0));
+			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		} else {
+			ga.part_id = (uint32_t)part;
+			ga.append = ctr_field(h, offsetof(RunCtrs, append));
+			TRY(set_ctr(h, offsetof(RunCtrs, append), 0ull));
+			ProfScope ps(h, "gen", (double)part_count[part] * 8.0 + (double)nseq * 24.0);
+			k_gen<GEN_APPEND><<<gen_grid, GEN_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		}
+		return XTN_OK;
+	};
 
-		bool sorted_layout = true;
-		if (region) {
-			// exact pass over the top 8 bucket bits, then the region scatter
-			// over the rest; buckets are read from their regions
-			StageScope sg(h, "sort");
-			uint32_t* hist = h->hist.as<uint32_t>();
-			if (!fused_hist) {
-				ProfScope ps(h, "sort_rec_hist", (double)nrec * 8.0);
-				k_radix_hist<<<(unsigned)rtiles, RS_NT, 0, s>>>(h->recs.as<uint64_t>(), nrec, hi - 8, hist, rtiles);
-				HIP_LAUNCH_CHECK(h);
+	// candidate pair capacity: the last run's count with a quarter of
+	// headroom (V / 16 before any run); regrown on overflow. The tail's
+	// launches are sized by it.
+	int64_t pcap = std::min<int64_t>(
+	    pmax, std::max<int64_t>(h->last_pairs > 0 ? h->last_pairs + h->last_pairs / 4 : (int64_t)V / 16, 1 << 16));
+	if ((size_t)pcap * sizeof(uint64_t) > h->pairs.cap)
+		TRY(ensure(h, h->pairs, sizeof(uint64_t) * (size_t)pcap));
+
+	// rounds over the smaller id of a pair, processed in ascending order
+	std::deque<std::pair<uint32_t, uint32_t>> rounds;
+	rounds.emplace_back(0u, (uint32_t)nseq);
+	int64_t out_total = 0, max_round = 0;
+	Grouping g1;
+	bool grouped = false;  // single chunk: the grouping is done once for every round
+	bool no_graph = false;  // a replay needed the host: the rest of this run is eager
+	TailPlan tp;
+	CtlHead head{};
+	while (!rounds.empty()) {
+		const uint32_t lo = rounds.front().first, hi_id = rounds.front().second;
+		bool split = false;
+		int64_t n_pairs = 0;
+		bool done = false;
+		if (nparts == 1 && h->tail_safe) {
+			if (!grouped) {
+				TRY(prologue());
+				TRY(plan_chunk((int64_t)V, g1));
+				TRY(issue_gen(0, g1));
+				TRY(issue_grouping(h, g1));
+				grouped = true;
 			}
-			uint32_t* dtot = nullptr;
-			TRY(digit_scan(h, hist, rtiles, &dtot));
-			k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
-			HIP_LAUNCH_CHECK(h);
-			{
-				ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)nrec);
-				k_radix_scatter<false, RS_NT, true><<<(unsigned)rtiles, RS_NT, 0, s>>>(
-				    h->recs.as<uint64_t>(), nullptr, h->recs_alt.as<uint64_t>(), nullptr, nrec, hi - 8, hist, rtiles);
-				HIP_LAUNCH_CHECK(h);
+			TRY(checked_candidates(h, g1, lo, hi_id, pmax, &pcap, &n_pairs, &split));
+		} else if (nparts == 1) {
+			// everything issued, one readback: the whole run of a single
+			// round is a graph once it has run the same way before
+			auto issue_all = [&]() -> int {
+				if (!grouped) {
+					TRY(prologue());
+					TRY(plan_chunk((int64_t)V, g1));
+					TRY(issue_gen(0, g1));
+					TRY(issue_grouping(h, g1));
+					grouped = true;
+				}
+				TRY(tail_prepare(h, pcap, idb, out_total, &tp));
+				TRY(zero_ctl(h, true, true));
+				TRY(issue_buckets(h, g1, lo, hi_id, pcap));
+				StageScope sg(h, "verify");
+				TRY(issue_tail(h, tp, idb, seq_map, row_base, set_of, out_total));
+				return XTN_OK;
+			};
+			const bool graph_ok = !grouped && !no_graph && h->cfg.no_graph == 0 && !h->cfg.profile &&
+			                      !h->cfg.verbose && v_known && lo == 0 && hi_id == (uint32_t)nseq && out_total == 0;
+			xtn_handle::GraphSig sig{};
+			if (graph_ok) {
+				sig.nseq = nseq;
+				sig.row_base = row_base;
+				sig.V = (int64_t)V;
+				sig.pcap = pcap;
+				sig.alloc_gen = h->alloc_gen;
+				sig.seq_map = seq_map;
+				sig.set_of = set_of;
+				sig.chars = h->chars.p;
+				sig.offs = h->offs.p;
+				sig.d = d;
+				sig.measure = h->cfg.measure;
+				sig.bucket_cap = bucket_cap;
 			}
-			k_fill_u64<<<grid_for(nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), nb / 2, 0ull);
-			HIP_LAUNCH_CHECK(h);
-			RegionArgs ra{};
-			ra.in = h->recs_alt.as<uint64_t>();
-			ra.out = h->recs.as<uint64_t>();
-			ra.ctl = h->ctl.as<uint32_t>();
-			ra.count = h->bcount.as<uint32_t>();
-			ra.stride = bucket_cap;
-			ra.shift = hi - B;
-			ra.fine_bits = B - 8;
-			{
-				ProfScope ps(h, "region", 16.0 * (double)nrec);
-				k_region_scatter<<<(unsigned)(rtiles + RS_BINS), RS_NT, 0, s>>>(ra);
-				HIP_LAUNCH_CHECK(h);
-			}
-			ba.rec = h->recs.as<uint64_t>();
-			ba.bcount = h->bcount.as<uint32_t>();
-			ba.bstride = bucket_cap;
-			ba.bstart = nullptr;
-			TRY(run_buckets());
-			if (rc.append > 0) {
-				// a bucket outgrew its region: redo the chunk on the exact
-				// two-pass sort of the coarse-sorted records
-				std::swap(h->recs, h->recs_alt);
-				ba.bcount = nullptr;
+			bool via_graph = false;
+			if (graph_ok && h->graph_exec && sig == h->graph_sig) {
+				HIP_TRY(h, hipGraphLaunch(h->graph_exec, s));
+				st.n_launches += h->graph_launches;
+				via_graph = true;
+			} else if (graph_ok && h->graph_want_set && sig == h->graph_want) {
+				// the previous run had this signature: capture this one (every
+				// buffer is already allocated; an allocation attempt during the
+				// capture abandons it)
+				const int64_t l0 = st.n_launches;
+				hipGraph_t graph = nullptr;
+				hipGraphExec_t exec = nullptr;
+				HIP_TRY(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+				h->capturing = true;
+				int rc = issue_all();
+				h->capturing = false;
+				hipError_t e = hipStreamEndCapture(s, &graph);
+				if (rc == XTN_OK && e == hipSuccess)
+					e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+				if (rc == XTN_OK && e == hipSuccess) {
+					if (h->graph_exec)
+						(void)hipGraphExecDestroy(h->graph_exec);
+					if (h->graph)
+						(void)hipGraphDestroy(h->graph);
+					h->graph = graph;
+					h->graph_exec = exec;
+					h->graph_sig = sig;
+					h->graph_launches = st.n_launches - l0;
+					HIP_TRY(h, hipGraphLaunch(exec, s));
+					via_graph = true;
+				} else {
+					// nothing of the capture ran: issue the run eagerly
+					(void)hipGetLastError();
+					if (graph)
+						(void)hipGraphDestroy(graph);
+					h->graph_want_set = false;
+					grouped = false;
+					prologue_done = false;
+					st.n_launches = l0;
+					TRY(issue_all());
+				}
 			} else {
-				sorted_layout = false;
+				TRY(issue_all());
+			}
+			TRY(read_head(h, &head));
+			const bool clean = !head.error && (int64_t)head.rc.n_cand <= pcap && head.rc.append == 0;
+			if (via_graph) {
+				st.graph_replays += 1;
+				if (!clean) {
+					// redo the run eagerly (the host's plan of the layout is
+					// rebuilt with it)
+					no_graph = true;
+					grouped = false;
+					prologue_done = false;
+					continue;
+				}
+			}
+			if (head.error) {
+				// a look-back wait hit its limit: this handle keeps to the
+				// host-checked tail from now on, and the round is redone
+				h->tail_safe = true;
+				continue;
+			}
+			if (clean) {
+				n_pairs = (int64_t)head.rc.n_cand;
+				done = true;
+				if (graph_ok && !via_graph) {
+					h->graph_want = sig;
+					h->graph_want_set = true;
+				}
+			} else {
+				TRY(checked_candidates(h, g1, lo, hi_id, pmax, &pcap, &n_pairs, &split));
+			}
+		} else {
+			TRY(set_ctr(h, offsetof(RunCtrs, n_cand), 0ull));
+			for (int part = 0; part < nparts && !split; ++part) {
+				const int64_t nrec = (int64_t)part_count[part];
+				if (nrec < 2)
+					continue;
+				Grouping g;
+				TRY(plan_chunk(nrec, g));
+				TRY(issue_gen(part, g));
+				TRY(issue_grouping(h, g));
+				TRY(checked_candidates(h, g, lo, hi_id, pmax, &pcap, &n_pairs, &split));
 			}
 		}
-		if (sorted_layout) {
-			if (B > 0) {
-				StageScope sg(h, "sort");
-				TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, nrec, hi - SB, hi, "sort_rec",
-				               fused_hist && !region));
+		if (split) {
+			// more candidates than a round holds: this id range becomes
+			// several, each a round of its own
+			const uint32_t span = hi_id - lo;
+			if (span <= 1)
+				return fail(h, XTN_ERR_CAPACITY, "the candidates of one sequence exceed the round budget (%lld)",
+				            (long long)pmax);
+			rounds.pop_front();
+			const uint32_t mid = lo + span / 2;
+			rounds.emplace_front(mid, hi_id);
+			rounds.emplace_front(lo, mid);
+			continue;
+		}
+		if (!done && h->tail_safe) {
+			StageScope sg(h, "verify");
+			TRY(tail_rts(h, n_pairs, idb, seq_map, row_base, set_of, out_total, &head));
+		} else if (!done) {
+			TRY(tail_prepare(h, pcap, idb, out_total, &tp));
+			TRY(set_ctr(h, offsetof(RunCtrs, n_cand), (unsigned long long)n_pairs));
+			TRY(zero_ctl(h, false, true));
+			{
+				StageScope sg(h, "verify");
+				TRY(issue_tail(h, tp, idb, seq_map, row_base, set_of, out_total));
 			}
-			ba.rec = h->recs.as<uint64_t>();
-			ba.bstart = h->bstart.as<uint32_t>();
-			TRY(run_buckets());
+			TRY(read_head(h, &head));
+			if (head.error) {
+				h->tail_safe = true;  // see above; the round is redone
+				continue;
+			}
 		}
-		n_pairs = (int64_t)rc.n_cand;
-		const int64_t n_ovf = (int64_t)rc.append;
-		if (n_ovf > 0) {
-			StageScope sg(h, "pairs");
-			std::vector<uint32_t> ids((size_t)n_ovf), bs((size_t)nb + 1);
-			HIP_TRY(h, hipMemcpyAsync(ids.data(), h->ovf.p, sizeof(uint32_t) * (size_t)n_ovf, hipMemcpyDeviceToHost, s));
-			HIP_TRY(h, hipMemcpyAsync(bs.data(), h->bstart.p, sizeof(uint32_t) * ((size_t)nb + 1),
-			                          hipMemcpyDeviceToHost, s));
-			HIP_TRY(h, hipStreamSynchronize(s));
-			for (uint32_t b : ids)
-				TRY(bucket_fallback(h, bs[b], (int64_t)bs[b + 1] - bs[b], rs, hi - B, idb, &n_pairs));
-		}
+		st.n_candidates += n_pairs;
+		st.n_unique += (int64_t)head.n_unique;
+		out_total += (int64_t)head.n_out;
+		max_round = std::max(max_round, n_pairs);
+		++st.n_rounds;
+		st.tail_checked = h->tail_safe ? 1 : 0;
+		rounds.pop_front();
 	}
-	st.n_candidates = n_pairs;
-	h->last_pairs = n_pairs;
-
-	// sort the candidates, keep the unique ones, verify those, keep the pairs
-	// within the distance; the last three steps preserve order
-	uint32_t nout = 0;
-	if (n_pairs > 0) {
-		StageScope sg(h, "verify");
-		TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)n_pairs));
-		TRY(radix_sort(h, h->pairs, h->pairs_alt, nullptr, nullptr, n_pairs, 0, 2 * idb, "sort_pairs"));
-		const int64_t nt = (n_pairs + SEL_TILE - 1) / SEL_TILE;  // unique step tiles
-		const int64_t nv = (n_pairs + SEL_NT - 1) / SEL_NT;      // verify step tiles
-		TRY(ensure(h, h->pos, sizeof(uint32_t) * (size_t)(nt + 1 + nv + 1)));
-		TRY(ensure(h, h->dist, (size_t)n_pairs));
-		uint32_t* ucnt = h->pos.as<uint32_t>();       // nt counts, total at [nt]
-		uint32_t* kcnt = ucnt + nt + 1;               // nv counts of kept pairs, total at [nv]
-		uint64_t* upairs = h->pairs_alt.as<uint64_t>();
-		{
-			ProfScope ps(h, "unique", (double)n_pairs * 16.0);
-			k_uniq_count<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt);
-			HIP_LAUNCH_CHECK(h);
-			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(ucnt, nt, ucnt + nt, nullptr, 1);
-			HIP_LAUNCH_CHECK(h);
-			k_uniq_write<<<(unsigned)nt, SEL_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_pairs, ucnt, upairs);
-			HIP_LAUNCH_CHECK(h);
-		}
-		{
-			ProfScope ps(h, "verify", 0.0);
-			auto verify = measure == 0 && d > 8 ? k_verify_count<true> : k_verify_count<false>;
-			verify<<<(unsigned)nv, SEL_NT, 0, s>>>(
-			    upairs, ucnt + nt, idb, h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), seq_map, row_base,
-			    set_of, d, measure, h->dist.as<int8_t>(), kcnt);
-			HIP_LAUNCH_CHECK(h);
-			// only the tiles holding unique pairs have counts to scan
-			k_scan_single<uint32_t><<<1, SC_NT, 0, s>>>(kcnt, nv, kcnt + nv, ucnt + nt, SEL_NT);
-			HIP_LAUNCH_CHECK(h);
-		}
-		// the outputs hold every unique pair (at most n_pairs), so the select
-		// is launched before the totals come back: the device does not wait
-		// on the host between verification and output; workgroups past the
-		// unique count exit at once
-		TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)n_pairs));
-		TRY(ensure(h, h->out_dist, (size_t)n_pairs));
-		{
-			ProfScope ps(h, "emit", (double)h->last_unique * 9.0 + (double)h->last_out * 9.0);
-			k_keep_write<<<(unsigned)nv, SEL_NT, 0, s>>>(upairs, ucnt + nt, idb, h->dist.as<int8_t>(), kcnt,
-			                                             h->out_pairs.as<int2>(), h->out_dist.as<int8_t>());
-			HIP_LAUNCH_CHECK(h);
-		}
-		uint32_t totals[2] = {0, 0};
-		HIP_TRY(h, hipMemcpyAsync(&totals[0], ucnt + nt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-		HIP_TRY(h, hipMemcpyAsync(&totals[1], kcnt + nv, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-		HIP_TRY(h, hipStreamSynchronize(s));
-		st.n_unique = (int64_t)totals[0];
-		h->last_unique = st.n_unique;
-		nout = totals[1];
-	}
-	HIP_TRY(h, hipStreamSynchronize(s));
-	h->out_len = nout;
-	h->last_out = nout;
-	st.n_output = h->out_len;
+	h->last_pairs = max_round;
+	h->last_unique = st.n_unique;
+	h->last_out = out_total;
+	h->out_len = out_total;
+	st.n_output = out_total;
 	st.ms_total = now_ms() - t0;
 	collect_stages(h);
 	collect_profile(h);
@@ -979,7 +1402,7 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 	uint32_t U = 0, ctl[2] = {0, 0};
 	HIP_TRY(h, hipMemcpyAsync(&U, h->pos.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
 	HIP_TRY(h, hipMemcpyAsync(ctl, h->scalar.p, sizeof(ctl), hipMemcpyDeviceToHost, s));
-	HIP_TRY(h, hipStreamSynchronize(s));
+	HIP_SYNC(h, s);
 	if (ctl[1] != 0xFFFFFFFFu)
 		return fail(h, XTN_ERR_ARG, "negative repertoire index at row %u", ctl[1]);
 	const int32_t R = (int32_t)ctl[0];
@@ -1008,7 +1431,7 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 		                                               rep_d, m);
 		HIP_LAUNCH_CHECK(h);
 	}
-	HIP_TRY(h, hipStreamSynchronize(s));
+	HIP_SYNC(h, s);
 	const double t_dedup = now_ms() - t0;
 
 	// neighbours among the unique strings
@@ -1029,7 +1452,7 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 	                        h->pos.as<uint32_t>() + RR));
 	uint32_t nout = 0;
 	HIP_TRY(h, hipMemcpyAsync(&nout, h->pos.as<uint32_t>() + RR, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-	HIP_TRY(h, hipStreamSynchronize(s));
+	HIP_SYNC(h, s);
 	TRY(ensure(h, h->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
 	TRY(ensure(h, h->ov_cnt, sizeof(unsigned long long) * (size_t)std::max<uint32_t>(nout, 1)));
 	k_matrix_emit<<<grid_for(RR, 256), 256, 0, s>>>(h->ov_present.as<uint32_t>(), h->pos.as<uint32_t>(),
@@ -1042,7 +1465,7 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 		HIP_TRY(h, hipMemcpyAsync(hp.data(), h->out_pairs.p, sizeof(xtn_int2) * nout, hipMemcpyDeviceToHost, s));
 		HIP_TRY(h, hipMemcpyAsync(hv.data(), h->ov_cnt.p, sizeof(uint64_t) * nout, hipMemcpyDeviceToHost, s));
 	}
-	HIP_TRY(h, hipStreamSynchronize(s));
+	HIP_SYNC(h, s);
 	collect_profile(h);
 	st.n_sequences = U;
 	st.n_output = nout;
@@ -1106,6 +1529,8 @@ int xtn_open(xtn_handle** out, const xtn_config* cfg) {
 		return XTN_ERR_HIP;
 	xtn_handle* h = new xtn_handle();
 	h->cfg = c;
+	if (const char* e = getenv("XTN_LOOKBACK_SPIN_LIMIT"))
+		h->spin_limit = (uint32_t)strtoul(e, nullptr, 10);
 	int ncu = 0;
 	if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c.device) == hipSuccess && ncu > 0)
 		h->n_cu = ncu;
@@ -1130,7 +1555,7 @@ void xtn_close(xtn_handle* h) {
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
 	                &h->ov_vals_alt, &h->ov_info, &h->ov_leader, &h->ov_ustart, &h->ov_umap,
 	                &h->ov_cnt, &h->ov_matrix, &h->ov_present, &h->bstart, &h->ovf, &h->set_of,
-	                &h->set_offs, &h->tseq};
+	                &h->set_offs, &h->tseq, &h->tail_state};
 	for (DBuf* b : bufs)
 		release(*b);
 	for (auto& pe : h->pending) {
@@ -1139,6 +1564,10 @@ void xtn_close(xtn_handle* h) {
 	}
 	for (hipEvent_t e : h->event_pool)
 		(void)hipEventDestroy(e);
+	if (h->graph_exec)
+		(void)hipGraphExecDestroy(h->graph_exec);
+	if (h->graph)
+		(void)hipGraphDestroy(h->graph);
 	(void)hipStreamDestroy(h->stream);
 	(void)hipStreamDestroy(h->copy_stream);
 	delete h;
@@ -1172,7 +1601,7 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 		HIP_TRY(h, hipMemcpyAsync(h->chars.p, chars, total, hipMemcpyHostToDevice, h->stream));
 	HIP_TRY(h, hipMemcpyAsync(h->offs.p, offsets, sizeof(uint32_t) * ((size_t)n + 1),
 	                          hipMemcpyHostToDevice, h->stream));
-	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	HIP_SYNC(h, h->stream);
 	h->n = n;
 	h->n_chars = total;
 	h->out_len = 0;
@@ -1209,7 +1638,7 @@ int xtn_copy_nn_output(xtn_handle* h, xtn_int2* pairs, int8_t* distances, int64_
 	HIP_TRY(h, hipMemcpyAsync(pairs, h->out_pairs.p, sizeof(xtn_int2) * (size_t)n,
 	                          hipMemcpyDeviceToHost, h->stream));
 	HIP_TRY(h, hipMemcpyAsync(distances, h->out_dist.p, (size_t)n, hipMemcpyDeviceToHost, h->stream));
-	HIP_TRY(h, hipStreamSynchronize(h->stream));
+	HIP_SYNC(h, h->stream);
 	return XTN_OK;
 }
 
@@ -1296,7 +1725,7 @@ static int copy_output(xtn_handle* h, std::vector<xtn_int2>& pairs, std::vector<
 		HIP_TRY(h, hipMemcpyAsync(pairs.data(), h->out_pairs.p, sizeof(xtn_int2) * (size_t)n,
 		                          hipMemcpyDeviceToHost, h->stream));
 		HIP_TRY(h, hipMemcpyAsync(dist.data(), h->out_dist.p, (size_t)n, hipMemcpyDeviceToHost, h->stream));
-		HIP_TRY(h, hipStreamSynchronize(h->stream));
+		HIP_SYNC(h, h->stream);
 	}
 	return XTN_OK;
 }
@@ -1334,6 +1763,11 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 		tot.ms_sort += st.ms_sort;
 		tot.ms_pairs += st.ms_pairs;
 		tot.ms_verify += st.ms_verify;
+		tot.n_rounds += st.n_rounds;
+		tot.n_launches += st.n_launches;
+		tot.n_syncs += st.n_syncs;
+		tot.graph_replays += st.graph_replays;
+		tot.tail_checked |= st.tail_checked;
 	};
 	std::vector<xtn_int2> hp;
 	std::vector<int8_t> hd;
@@ -1545,7 +1979,7 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 		                                          w->pairs.as<uint64_t>(), cap, &w->ctrs.as<RunCtrs>()->n_cand);
 		HIP_LAUNCH_CHECK(w);
 		HIP_TRY(w, hipMemcpyAsync(&found, &w->ctrs.as<RunCtrs>()->n_cand, sizeof(found), hipMemcpyDeviceToHost, s));
-		HIP_TRY(w, hipStreamSynchronize(s));
+		HIP_SYNC(w, s);
 		if ((int64_t)found <= cap)
 			break;
 		if (found > (unsigned long long)INT32_MAX)
@@ -1570,7 +2004,7 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	TRY(scan_excl<uint32_t>(w, w->flags.as<uint32_t>(), w->pos.as<uint32_t>(), np, w->pos.as<uint32_t>() + np));
 	uint32_t nout = 0;
 	HIP_TRY(w, hipMemcpyAsync(&nout, w->pos.as<uint32_t>() + np, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-	HIP_TRY(w, hipStreamSynchronize(s));
+	HIP_SYNC(w, s);
 	TRY(ensure(w, w->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
 	TRY(ensure(w, w->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
 	k_emit_nn<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->flags.as<uint32_t>(),
@@ -1584,7 +2018,7 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 		                          hipMemcpyDeviceToHost, s));
 		HIP_TRY(w, hipMemcpyAsync(res->dist.data(), w->out_dist.p, nout, hipMemcpyDeviceToHost, s));
 	}
-	HIP_TRY(w, hipStreamSynchronize(s));
+	HIP_SYNC(w, s);
 	return XTN_OK;
 }
 
@@ -1686,7 +2120,7 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 		TRY(scan_excl<uint64_t>(h, h->voff.as<uint64_t>(), h->voff.as<uint64_t>(), n, h->voff.as<uint64_t>() + n));
 		uint64_t V = 0;
 		HIP_TRY(h, hipMemcpyAsync(&V, h->voff.as<uint64_t>() + n, sizeof(V), hipMemcpyDeviceToHost, s));
-		HIP_TRY(h, hipStreamSynchronize(s));
+		HIP_SYNC(h, s);
 		if (V > (uint64_t)INT32_MAX)
 			return fail(h, XTN_ERR_CAPACITY, "index of %llu records exceeds 2^31", (unsigned long long)V);
 		TRY(ensure(h, h->recs, sizeof(uint64_t) * V));
@@ -1710,7 +2144,7 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 		k_dir_bounds<<<grid_for(((int64_t)1 << DIR_BITS) + 1, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), (int64_t)V,
 		                                                                         h->bstart.as<uint32_t>());
 		HIP_LAUNCH_CHECK(h);
-		HIP_TRY(h, hipStreamSynchronize(s));
+		HIP_SYNC(h, s);
 		sv->v_index = (int64_t)V;
 		sv->index = h->recs.as<uint64_t>();
 		sv->dir = h->bstart.as<uint32_t>();

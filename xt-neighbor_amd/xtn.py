@@ -67,7 +67,9 @@ class Config(ctypes.Structure):
                 ("verbose", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("max_records_per_chunk", ctypes.c_int64),
                 ("output_chunk_pairs", ctypes.c_int64),
-                ("profile", ctypes.c_int32), ("bucket_capacity", ctypes.c_int32)]
+                ("profile", ctypes.c_int32), ("bucket_capacity", ctypes.c_int32),
+                ("no_graph", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("max_pairs_per_round", ctypes.c_int64)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -84,7 +86,9 @@ class Stats(ctypes.Structure):
                 ("n_output", ctypes.c_int64), ("ms_total", ctypes.c_double),
                 ("ms_generate", ctypes.c_double), ("ms_sort", ctypes.c_double),
                 ("ms_pairs", ctypes.c_double), ("ms_verify", ctypes.c_double),
-                ("ms_output", ctypes.c_double)]
+                ("ms_output", ctypes.c_double), ("n_rounds", ctypes.c_int64),
+                ("tail_checked", ctypes.c_int64), ("n_launches", ctypes.c_int64),
+                ("n_syncs", ctypes.c_int64), ("graph_replays", ctypes.c_int64)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Output))
@@ -291,7 +295,8 @@ class Engine:
 
     def __init__(self, distance: int = 1, measure: int | str = LEVENSHTEIN, device: int = 0,
                  max_records_per_chunk: int = 0, output_chunk_pairs: int = 0,
-                 profile: bool = False, verbose: bool = False, bucket_capacity: int = 0):
+                 profile: bool = False, verbose: bool = False, bucket_capacity: int = 0,
+                 max_pairs_per_round: int = 0, no_graph: bool = False):
         self.lib = load_library()
         self.cfg = Config()
         self.lib.xtn_config_init(ctypes.byref(self.cfg))
@@ -303,6 +308,8 @@ class Engine:
         self.cfg.profile = 1 if profile else 0
         self.cfg.verbose = 1 if verbose else 0
         self.cfg.bucket_capacity = bucket_capacity
+        self.cfg.max_pairs_per_round = max_pairs_per_round
+        self.cfg.no_graph = 1 if no_graph else 0
         self.h = ctypes.c_void_p()
         st = self.lib.xtn_open(ctypes.byref(self.h), ctypes.byref(self.cfg))
         if st != 0:
@@ -355,7 +362,7 @@ class Engine:
         for k, v in kw.items():
             if k == "measure":
                 v = {"leven": LEVENSHTEIN, "hamming": HAMMING}.get(v, v)
-            if k == "profile":
+            if k in ("profile", "no_graph"):
                 v = 1 if v else 0
             setattr(self.cfg, k, v)
         self._check(self.lib.xtn_set_config(self.h, ctypes.byref(self.cfg)))
