@@ -144,10 +144,16 @@ def host_boundary(eng, chars, offsets, runs=3):
             "runs": runs}
 
 
-def served(n_index, n_queries, rate_hz, seed):
-    """query sets (log-uniform 1k..100k rows, 30% planted relatives of index
-    rows) arriving as a Poisson process against a resident 10M-row index,
-    Levenshtein d=1; latency is submit to completion"""
+def served(n_index, n_queries, seed, plain_queries=4):
+    """Query sets (log-uniform 1k..100k rows, 30% planted relatives of index
+    rows) against a resident index of n_index rows, Levenshtein d=1; latency
+    is submit to completion. Capacity: every query submitted at once, rows
+    per second of wall time until the last completes. Then Poisson arrivals
+    at 50, 80 and 100% of that rate (queries/s = capacity / mean rows):
+    p50/p99 latency and the rows/s sustained. Beside it, the same query sets
+    one at a time through the plain pipeline, which has no resident index:
+    one NN run over index + query (device time, input already resident),
+    rows/s = query rows / run time."""
     import threading
 
     ichars, ioffs = xtn.synth_nn(seed, n_index, 15.0, 2.0, 8, 22, 0.25, 1)
@@ -155,43 +161,76 @@ def served(n_index, n_queries, rate_hz, seed):
     sizes = np.exp(rng.uniform(np.log(1000), np.log(100000), n_queries)).astype(int)
     queries = [xtn.synth_queries(seed * 1000 + k, ichars, ioffs, int(sz), planted=0.3)
                for k, sz in enumerate(sizes)]
+    nq_rows = int(sizes.sum())
     t_build = time.perf_counter()
     srv = xtn.Server(ichars, ioffs, distance=1, slots=4, max_query_rows=1 << 17)
     build_ms = (time.perf_counter() - t_build) * 1e3
-    # warm-up: one query through every slot
-    for t in [srv.submit(*queries[k % n_queries]) for k in range(4)]:
+    for t in [srv.submit(*queries[k % n_queries]) for k in range(8)]:  # warm-up, every slot
         srv.wait(t)
-    gaps = rng.exponential(1.0 / rate_hz, n_queries)
-    tickets = []
-    lat = [0.0] * n_queries
-    pairs = [0] * n_queries
-    done_at = [0.0] * n_queries
 
-    def waiter():
+    def run(gaps):
+        tickets = [None] * n_queries
+        lat = [0.0] * n_queries
+        done_at = [0.0] * n_queries
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for k in range(n_queries):
+                ready.acquire()
+                _, _, l = srv.wait(tickets[k])
+                lat[k], done_at[k] = l, time.perf_counter()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        next_t = t0
         for k in range(n_queries):
-            while len(tickets) <= k:
-                time.sleep(0.0002)
-            p, _, l = srv.wait(tickets[k])
-            lat[k], pairs[k], done_at[k] = l, len(p), time.perf_counter()
+            if gaps is not None:
+                next_t += gaps[k]
+                delay = next_t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            tickets[k] = srv.submit(*queries[k])
+            ready.release()
+        th.join()
+        wall = max(done_at) - t0
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "seqs_per_s": nq_rows / wall, "wall_s": wall}
 
-    th = threading.Thread(target=waiter)
-    th.start()
-    t0 = time.perf_counter()
-    next_t = t0
-    for k in range(n_queries):
-        next_t += gaps[k]
-        delay = next_t - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        tickets.append(srv.submit(*queries[k]))
-    th.join()
-    wall = max(done_at) - t0
+    burst = run(None)
+    capacity = burst["seqs_per_s"]
+    loads = {}
+    for frac in (0.5, 0.8, 1.0):
+        rate = capacity * frac / (nq_rows / n_queries)  # queries per second
+        gaps = rng.exponential(1.0 / rate, n_queries)
+        r = run(gaps)
+        r["arrival_rate_hz"] = rate
+        loads[f"{int(frac * 100)}pct"] = r
     srv.close()
-    nq = int(sizes.sum())
-    return {"index_seqs": n_index, "queries": n_queries, "query_seqs": nq, "arrival_rate_hz": rate_hz,
-            "slots": 4, "index_build_ms": build_ms, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "seqs_per_s": nq / wall, "cross_pairs": int(sum(pairs)),
-            "wall_s": wall}
+    # the plain pipeline: index + query as one input, one NN run per query
+    plain = []
+    with xtn.Engine(distance=1) as eng:
+        for k in range(min(plain_queries, n_queries)):
+            qc, qo = queries[k]
+            chars = np.concatenate([ichars, qc])
+            offs = np.concatenate([ioffs, qo[1:].astype(np.int64) + int(ioffs[-1])]).astype(np.uint32)
+            eng.set_sequences(chars=chars, offsets=offs)
+            eng.nn_resident()  # warm-up
+            hip_sync()
+            t0 = time.perf_counter()
+            eng.nn_resident()
+            hip_sync()
+            plain.append((len(qo) - 1, time.perf_counter() - t0))
+    plain_rows = sum(r for r, _ in plain)
+    plain_s = sum(t for _, t in plain)
+    return {"index_seqs": n_index, "queries": n_queries, "query_seqs": nq_rows, "slots": 4,
+            "index_build_ms": build_ms, "capacity_seqs_per_s": capacity, "burst": burst, "load": loads,
+            "p50_ms": loads["80pct"]["p50_ms"], "p99_ms": loads["80pct"]["p99_ms"],
+            "plain_per_query": {"queries": len(plain), "query_seqs": plain_rows,
+                                "ms_per_query": plain_s / max(len(plain), 1) * 1e3,
+                                "seqs_per_s": plain_rows / plain_s if plain_s > 0 else None,
+                                "note": "one NN run over index + query per query set (no resident index)"},
+            "capacity_vs_plain": capacity / (plain_rows / plain_s) if plain_s > 0 else None}
 
 
 def end_to_end(chars, offsets, d, measure):
@@ -278,6 +317,57 @@ def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int
     return out
 
 
+def timed_runs(eng, steps):
+    """ms per nn_resident run (median of `steps` after one warm-up) and the stats"""
+    eng.nn_resident()
+    times = []
+    for _ in range(steps):
+        hip_sync()
+        t0 = time.perf_counter()
+        eng.nn_resident()
+        hip_sync()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times) * 1e3, eng.stats()
+
+
+def chunked_leg(steps: int = 5) -> dict:
+    """5M synthetic CDR3 (lengths ~N(15,2.5) in [6,30], 25% planted 1-edit
+    relatives), d=1, Levenshtein and Hamming: one key-space chunk against the
+    key space forced into chunks of at most 20M records (each chunk
+    regenerates its part of the records, so the chunked run pays the
+    generator once per chunk plus a counting pass)"""
+    n = 5_000_000
+    chars, offsets = xtn.synth_nn(3, n, 15.0, 2.5, 6, 30, 0.25, 1)
+    out = {"n_seqs": n, "distance": 1}
+    with xtn.Engine(distance=1) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        for measure in ("leven", "hamming"):
+            for label, chunk in (("one_chunk", 0), ("chunks", 20_000_000)):
+                eng.configure(measure=measure, max_records_per_chunk=chunk)
+                ms, st = timed_runs(eng, steps)
+                out[f"{measure}_{label}"] = {"ms": ms, "seqs_per_s": n / (ms / 1e3), "chunks": st["n_chunks"],
+                                             "variants": st["n_variants"], "output_pairs": st["n_output"],
+                                             "syncs": st["n_syncs"], "launches": st["n_launches"]}
+    return out
+
+
+def distance_leg(steps: int = 5) -> dict:
+    """100k synthetic CDR3 at d = 1..5 (the reference publishes 100k at
+    d = 1..5, data/gpu_dist_benchmark.csv); d >= 4 takes the generator's
+    colex unranking"""
+    n = 100_000
+    chars, offsets = xtn.synth_nn(7, n, planted=0.25, max_edits=2)
+    out = {}
+    with xtn.Engine(distance=1) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        for d in range(1, 6):
+            eng.configure(distance=d)
+            ms, st = timed_runs(eng, steps)
+            out[f"d{d}"] = {"ms": ms, "seqs_per_s": n / (ms / 1e3), "variants": st["n_variants"],
+                            "unique_pairs": st["n_unique"], "output_pairs": st["n_output"]}
+    return out
+
+
 def copy_probe(nbytes: int = 1 << 30) -> dict:
     """a plain copy kernel of nbytes on the same GPU: the practical HBM ceiling"""
     probe = ctypes.CDLL(os.path.join(REPO, "xt-neighbor_amd", "lib", "libxtn_probe.so"))
@@ -318,10 +408,11 @@ def main():
     ap.add_argument("--batched-sets", type=int, default=1000)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-overlap", action="store_true")
+    ap.add_argument("--no-chunked", action="store_true")
+    ap.add_argument("--no-distances", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--served-index", type=int, default=10_000_000)
     ap.add_argument("--served-queries", type=int, default=64)
-    ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -429,8 +520,14 @@ def main():
         result["end_to_end_cli"] = end_to_end(chars, offsets, d, measure)
         progress("CLI end to end done")
     if not args.no_served:
-        result["served"] = served(args.served_index, args.served_queries, args.served_rate, 5)
+        result["served"] = served(args.served_index, args.served_queries, 5)
         progress("served done")
+    if not args.no_chunked:
+        result["chunked_5M_d1"] = chunked_leg()
+        progress("chunked 5M done")
+    if not args.no_distances:
+        result["distances_100k"] = distance_leg()
+        progress("distance scaling done")
     if not args.no_overlap:
         result["overlap"] = overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e)
         progress("overlap done")

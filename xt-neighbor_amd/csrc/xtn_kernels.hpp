@@ -1651,21 +1651,6 @@ __global__ __launch_bounds__(SEL_NT) void k_keep_write(const uint64_t* __restric
 	}
 }
 
-__global__ __launch_bounds__(256) void k_emit_nn(const uint64_t* __restrict__ pairs, int64_t n,
-                                                 int idb, const uint32_t* __restrict__ flags,
-                                                 const uint32_t* __restrict__ pos,
-                                                 const int8_t* __restrict__ dist,
-                                                 int2* __restrict__ out_pairs,
-                                                 int8_t* __restrict__ out_dist) {
-	const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (q >= n || !flags[q])
-		return;
-	const uint64_t p = pairs[q];
-	const uint32_t o = pos[q];
-	out_pairs[o] = make_int2((int)(p >> idb), (int)(p & ((1ull << idb) - 1ull)));
-	out_dist[o] = dist[q];
-}
-
 // set_of[r] = the query set holding row r0 + r: the largest k with
 // offs[k] <= r0 + r (empty sets share their offset with the next set)
 __global__ __launch_bounds__(256) void k_set_of(const uint32_t* __restrict__ offs, int32_t n_sets,

@@ -112,6 +112,9 @@ struct xtn_handle {
 	int64_t graph_launches = 0;  // kernels the graph holds
 	bool capturing = false;
 	int64_t alloc_gen = 0;  // bumped by every device allocation of the handle
+	void* pin = nullptr;  // pinned upload staging of a server worker
+	size_t pin_bytes = 0;
+	double serve_ratio = 0;  // a server worker's candidates per query record so far
 	bool tail_safe = false;  // a look-back wait hit its limit once: host-checked tail from then on
 	uint32_t spin_limit = LB_SPIN_LIMIT;  // XTN_LOOKBACK_SPIN_LIMIT overrides (tests of the fallback)
 	size_t free_cache = 0;   // hipMemGetInfo's free bytes, re-read after an allocation
@@ -623,7 +626,8 @@ int tail_prepare(xtn_handle* h, int64_t pcap, int idb, int64_t out_keep, TailPla
 // counter), outputs written from out_pairs[out_off]; the tail block must be
 // zero (zero_ctl)
 int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_map, uint32_t row_base,
-               const uint32_t* set_of, int64_t out_off) {
+               const uint32_t* set_of, int64_t out_off, const uint8_t* chars_b = nullptr,
+               const uint32_t* offs_b = nullptr) {
 	hipStream_t s = h->stream;
 	CtlBlock* cb = ctl_block(h);
 	const unsigned long long* n_dev = &cb->rc.n_cand;
@@ -661,6 +665,8 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 	sa.offs = h->offs.as<uint32_t>();
 	sa.seq_map = seq_map;
 	sa.set_of = set_of;
+	sa.chars_b = chars_b;
+	sa.offs_b = offs_b;
 	sa.out_pairs = h->out_pairs.as<int2>() + out_off;
 	sa.out_dist = h->out_dist.as<int8_t>() + out_off;
 	sa.ctl = &cb->tc;
@@ -1564,6 +1570,8 @@ void xtn_close(xtn_handle* h) {
 	}
 	for (hipEvent_t e : h->event_pool)
 		(void)hipEventDestroy(e);
+	if (h->pin)
+		(void)hipHostFree(h->pin);
 	if (h->graph_exec)
 		(void)hipGraphExecDestroy(h->graph_exec);
 	if (h->graph)
@@ -1908,6 +1916,7 @@ struct xtn_server {
 	int64_t v_index = 0;
 	const uint64_t* index = nullptr;
 	const uint32_t* dir = nullptr;
+	int dir_bits = 8;
 	std::vector<xtn_handle*> workers;
 	std::vector<std::thread> threads;
 	std::mutex mu;
@@ -1932,8 +1941,12 @@ int sv_fail(xtn_server* sv, int code, const std::string& msg) {
 	return code;
 }
 
-// one query on a worker handle: variants, index lookups, candidate sort and
-// unique, verification, compaction, download
+// one query on a worker handle: upload from the worker's pinned staging,
+// variants, index lookups appending candidate pairs, then the search's tail
+// (sort, unique, verification against the index strings, ordered select)
+// issued back to back; the host reads the counters once and copies the
+// output. A candidate buffer that was too small is regrown and the lookups
+// redone (the records are still in place).
 int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResult* res) {
 	hipStream_t s = w->stream;
 	const int32_t n = rq.n;
@@ -1942,15 +1955,33 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	const int idb = sv->idb;
 	const int d = sv->cfg.distance;
 	const uint32_t total = rq.offs[n];
+	const size_t up = (size_t)total + sizeof(uint32_t) * ((size_t)n + 1);
+	if (up > w->pin_bytes) {
+		if (w->pin)
+			(void)hipHostFree(w->pin);
+		w->pin = nullptr;
+		w->pin_bytes = 0;
+		if (hipHostMalloc(&w->pin, up + up / 4, hipHostMallocDefault) != hipSuccess) {
+			(void)hipGetLastError();
+			w->pin = nullptr;
+			return fail(w, XTN_ERR_NOMEM, "pinned staging allocation failed");
+		}
+		w->pin_bytes = up + up / 4;
+	}
+	// the previous query's upload has completed: its output copy, after it
+	// on the same stream, was waited on
+	char* pin = static_cast<char*>(w->pin);
+	uint32_t* pin_offs = reinterpret_cast<uint32_t*>(pin + (((size_t)total + 3) & ~(size_t)3));
+	memcpy(pin, rq.chars.data(), total);
+	memcpy(pin_offs, rq.offs.data(), sizeof(uint32_t) * ((size_t)n + 1));
 	TRY(ensure(w, w->chars, (size_t)total + 16));
 	TRY(ensure(w, w->offs, sizeof(uint32_t) * ((size_t)n + 1)));
 	if (total)
-		HIP_TRY(w, hipMemcpyAsync(w->chars.p, rq.chars.data(), total, hipMemcpyHostToDevice, s));
-	HIP_TRY(w, hipMemcpyAsync(w->offs.p, rq.offs.data(), sizeof(uint32_t) * ((size_t)n + 1),
-	                          hipMemcpyHostToDevice, s));
+		HIP_TRY(w, hipMemcpyAsync(w->chars.p, pin, total, hipMemcpyHostToDevice, s));
+	HIP_TRY(w, hipMemcpyAsync(w->offs.p, pin_offs, sizeof(uint32_t) * ((size_t)n + 1), hipMemcpyHostToDevice, s));
 	TRY(ensure(w, w->pk, sizeof(uint64_t) * 2 * (size_t)n));
 	TRY(ensure(w, w->voff, sizeof(uint64_t) * ((size_t)n + 1)));
-	TRY(ensure(w, w->ctrs, sizeof(RunCtrs)));
+	TRY(ensure(w, w->ctrs, sizeof(CtlBlock)));
 	k_pack<<<grid_for(n, 256), 256, 0, s>>>(w->chars.as<uint8_t>(), w->offs.as<uint32_t>(), nullptr, 0, nullptr,
 	                                         n, d, w->pk.as<uint64_t>(), w->voff.as<uint64_t>());
 	HIP_LAUNCH_CHECK(w);
@@ -1966,59 +1997,47 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	ga.V = V;
 	ga.out = w->recs.as<uint64_t>();
 	ga.cap = V;
-	if (V > 0)
-		TRY(tile_seq_table(w, s, n, (uint64_t)V, &ga.tile_seq));
+	TRY(tile_seq_table(w, s, n, (uint64_t)V, &ga.tile_seq));
 	k_gen<GEN_DIRECT><<<grid_for(V, GEN_TILE), GEN_NT, 0, s>>>(ga);
 	HIP_LAUNCH_CHECK(w);
-	int64_t cap = std::max<int64_t>({w->last_pairs + w->last_pairs / 4, V / 2, 1 << 16});
-	unsigned long long found = 0;
+	// candidates per record seen so far on this worker, with headroom
+	const double ratio = w->serve_ratio > 0 ? w->serve_ratio : 1.0;
+	int64_t pcap = std::max<int64_t>((int64_t)((double)V * ratio * 1.5) + 1024, 1 << 16);
+	pcap = std::min<int64_t>(pcap, (int64_t)LB_VAL);
+	CtlHead head{};
+	TailPlan tp;
 	for (int attempt = 0; attempt < 2; ++attempt) {
-		TRY(ensure(w, w->pairs, sizeof(uint64_t) * (size_t)cap));
-		k_fill_u64<<<1, 256, 0, s>>>(&w->ctrs.as<RunCtrs>()->n_cand, 1, 0ull);
-		k_match<<<grid_for(V, 256), 256, 0, s>>>(w->recs.as<uint64_t>(), V, sv->index, sv->dir, idb,
-		                                          w->pairs.as<uint64_t>(), cap, &w->ctrs.as<RunCtrs>()->n_cand);
+		TRY(ensure(w, w->pairs, sizeof(uint64_t) * (size_t)pcap));
+		TRY(tail_prepare(w, pcap, idb, 0, &tp));
+		TRY(zero_ctl(w, true, true));
+		k_match<<<grid_for(V, MATCH_NT), MATCH_NT, 0, s>>>(w->recs.as<uint64_t>(), V, sv->index, sv->dir,
+		                                                   sv->dir_bits, idb, w->pairs.as<uint64_t>(), pcap,
+		                                                   &ctl_block(w)->rc.n_cand);
 		HIP_LAUNCH_CHECK(w);
-		HIP_TRY(w, hipMemcpyAsync(&found, &w->ctrs.as<RunCtrs>()->n_cand, sizeof(found), hipMemcpyDeviceToHost, s));
-		HIP_SYNC(w, s);
-		if ((int64_t)found <= cap)
+		TRY(issue_tail(w, tp, idb, nullptr, 0, nullptr, 0, sv->base->chars.as<uint8_t>(),
+		               sv->base->offs.as<uint32_t>()));
+		TRY(read_head(w, &head));
+		if (head.error)
+			return fail(w, XTN_ERR_HIP, "a look-back wait of the query's candidate tail exceeded its limit");
+		if ((int64_t)head.rc.n_cand <= pcap)
 			break;
-		if (found > (unsigned long long)INT32_MAX)
-			return fail(w, XTN_ERR_CAPACITY, "query candidates exceed 2^31");
-		cap = (int64_t)found + (int64_t)found / 4;
+		if (attempt == 1 || head.rc.n_cand > (unsigned long long)LB_VAL)
+			return fail(w, XTN_ERR_CAPACITY, "query candidates exceed the per-query limit (%llu)",
+			            (unsigned long long)head.rc.n_cand);
+		pcap = (int64_t)head.rc.n_cand + (int64_t)head.rc.n_cand / 4;
+		pcap = std::min<int64_t>(pcap, (int64_t)LB_VAL);
 	}
-	const int64_t np = (int64_t)found;
-	w->last_pairs = np;
-	if (np == 0)
-		return XTN_OK;
-	TRY(ensure(w, w->pairs_alt, sizeof(uint64_t) * (size_t)np));
-	TRY(radix_sort(w, w->pairs, w->pairs_alt, nullptr, nullptr, np, 0, 2 * idb, "sort_pairs"));
-	TRY(ensure(w, w->flags, sizeof(uint32_t) * (size_t)np));
-	TRY(ensure(w, w->pos, sizeof(uint32_t) * ((size_t)np + 1)));
-	TRY(ensure(w, w->dist, (size_t)np));
-	auto verify = sv->cfg.measure == 0 && d > 8 ? k_verify_cross<true> : k_verify_cross<false>;
-	verify<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->chars.as<uint8_t>(),
-	                                          w->offs.as<uint32_t>(), sv->base->chars.as<uint8_t>(),
-	                                          sv->base->offs.as<uint32_t>(), d, sv->cfg.measure,
-	                                          w->flags.as<uint32_t>(), w->dist.as<int8_t>());
-	HIP_LAUNCH_CHECK(w);
-	TRY(scan_excl<uint32_t>(w, w->flags.as<uint32_t>(), w->pos.as<uint32_t>(), np, w->pos.as<uint32_t>() + np));
-	uint32_t nout = 0;
-	HIP_TRY(w, hipMemcpyAsync(&nout, w->pos.as<uint32_t>() + np, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-	HIP_SYNC(w, s);
-	TRY(ensure(w, w->out_pairs, sizeof(int2) * (size_t)std::max<uint32_t>(nout, 1)));
-	TRY(ensure(w, w->out_dist, (size_t)std::max<uint32_t>(nout, 1)));
-	k_emit_nn<<<grid_for(np, 256), 256, 0, s>>>(w->pairs.as<uint64_t>(), np, idb, w->flags.as<uint32_t>(),
-	                                             w->pos.as<uint32_t>(), w->dist.as<int8_t>(),
-	                                             w->out_pairs.as<int2>(), w->out_dist.as<int8_t>());
-	HIP_LAUNCH_CHECK(w);
-	res->pairs.resize(nout);
-	res->dist.resize(nout);
+	const double r = (double)head.rc.n_cand / (double)std::max<int64_t>(V, 1);
+	w->serve_ratio = w->serve_ratio > 0 ? std::max(r, 0.75 * w->serve_ratio + 0.25 * r) : std::max(r, 0.01);
+	const int64_t nout = (int64_t)head.n_out;
+	res->pairs.resize((size_t)nout);
+	res->dist.resize((size_t)nout);
 	if (nout) {
-		HIP_TRY(w, hipMemcpyAsync(res->pairs.data(), w->out_pairs.p, sizeof(xtn_int2) * nout,
+		HIP_TRY(w, hipMemcpyAsync(res->pairs.data(), w->out_pairs.p, sizeof(xtn_int2) * (size_t)nout,
 		                          hipMemcpyDeviceToHost, s));
-		HIP_TRY(w, hipMemcpyAsync(res->dist.data(), w->out_dist.p, nout, hipMemcpyDeviceToHost, s));
+		HIP_TRY(w, hipMemcpyAsync(res->dist.data(), w->out_dist.p, (size_t)nout, hipMemcpyDeviceToHost, s));
+		HIP_SYNC(w, s);
 	}
-	HIP_SYNC(w, s);
 	return XTN_OK;
 }
 
@@ -2105,7 +2124,7 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 	}
 	sv->n_index = n;
 	sv->idb = std::max(bits_for(std::max(n - 1, 1)), bits_for(std::max(cfg.max_query_rows - 1, 1)));
-	if (2 * sv->idb > 62 || 64 - sv->idb < DIR_BITS + 8) {
+	if (2 * sv->idb > 62 || 64 - sv->idb < DIR_BITS_MAX + 8) {
 		xtn_server_close(sv);
 		return XTN_ERR_CAPACITY;
 	}
@@ -2140,9 +2159,14 @@ int xtn_server_open(xtn_server** out, const xtn_server_config* cfg_in, const cha
 		HIP_LAUNCH_CHECK(h);
 		TRY(radix_sort(h, h->recs, h->recs_alt, nullptr, nullptr, (int64_t)V, sv->idb, 64, "index_sort"));
 		release(h->recs_alt);
-		TRY(ensure(h, h->bstart, sizeof(uint32_t) * (((size_t)1 << DIR_BITS) + 1)));
-		k_dir_bounds<<<grid_for(((int64_t)1 << DIR_BITS) + 1, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), (int64_t)V,
-		                                                                         h->bstart.as<uint32_t>());
+		// directory bits: about eight index records per entry
+		int db = 8;
+		while (db < DIR_BITS_MAX && ((int64_t)V >> (db + 3)) > 0)
+			++db;
+		sv->dir_bits = db;
+		TRY(ensure(h, h->bstart, sizeof(uint32_t) * (((size_t)1 << db) + 1)));
+		k_dir_fill<<<grid_for((int64_t)V + 1, 256), 256, 0, s>>>(h->recs.as<uint64_t>(), (int64_t)V, db,
+		                                                          h->bstart.as<uint32_t>());
 		HIP_LAUNCH_CHECK(h);
 		HIP_SYNC(h, s);
 		sv->v_index = (int64_t)V;

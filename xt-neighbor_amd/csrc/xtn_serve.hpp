@@ -4,10 +4,13 @@
 // xtn_inner.cu:329-358, kept in HBM).
 //
 //   index:  rec = hash(variant)[63:idb] | index row, sorted by the hash bits,
-//           plus a directory dir[b] = first record whose top DIR_BITS are >= b
+//           plus a directory dir[b] = first record whose top dir_bits are
+//           >= b (about eight records per directory entry)
 //   query:  the query's variant records, each looked up in the index
-//           (directory + binary search), one candidate pair per match
-//           p = query row << idb | index row, then sort, unique, verification
+//           (directory, then a short binary search), one candidate pair per
+//           match p = query row << idb | index row; the candidates go
+//           through the same tail as a search (sort, unique, verification
+//           of query string against index string, ordered select)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,42 +20,46 @@
 
 namespace xtn {
 
-constexpr int DIR_BITS = 20;
+constexpr int DIR_BITS_MAX = 24;
 
-__global__ __launch_bounds__(256) void k_dir_bounds(const uint64_t* __restrict__ idx, int64_t n,
-                                                    uint32_t* __restrict__ dir) {
-	const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (b > (1ll << DIR_BITS))
+// Directory of a sorted index of n records over its top dir_bits bits, in
+// one pass: the thread of record r writes the entries b in (top(r - 1),
+// top(r)], so each of the 2^dir_bits + 1 entries is written exactly once
+// (record n stands for "past the end", top = 2^dir_bits).
+__global__ __launch_bounds__(256) void k_dir_fill(const uint64_t* __restrict__ idx, int64_t n, int dir_bits,
+                                                  uint32_t* __restrict__ dir) {
+	const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	if (r > n)
 		return;
-	int64_t lo = 0, hi = n;
-	while (lo < hi) {
-		const int64_t mid = (lo + hi) >> 1;
-		if ((int64_t)(idx[mid] >> (64 - DIR_BITS)) < b)
-			lo = mid + 1;
-		else
-			hi = mid;
-	}
-	dir[b] = (uint32_t)lo;
+	const int sh = 64 - dir_bits;
+	const int64_t prev = r == 0 ? -1 : (int64_t)(idx[r - 1] >> sh);
+	const int64_t cur = r == n ? ((int64_t)1 << dir_bits) : (int64_t)(idx[r] >> sh);
+	for (int64_t b = prev + 1; b <= cur; ++b)
+		dir[b] = (uint32_t)r;
 }
 
-// one thread per query record: the index records with the same hash bits
-__global__ __launch_bounds__(256) void k_match(const uint64_t* __restrict__ qrec, int64_t nq,
-                                               const uint64_t* __restrict__ idx,
-                                               const uint32_t* __restrict__ dir, int idb,
-                                               uint64_t* __restrict__ pairs, int64_t cap,
-                                               unsigned long long* __restrict__ ctr) {
-	__shared__ uint32_t s_tmp[256 / WAVE + 1];
+constexpr int MATCH_NT = 1024;  // one append per workgroup: fewer atomics on the counter
+
+// one thread per query record: the index records with the same hash bits,
+// appended as candidate pairs (one atomic per workgroup)
+__global__ __launch_bounds__(MATCH_NT) void k_match(const uint64_t* __restrict__ qrec, int64_t nq,
+                                                    const uint64_t* __restrict__ idx,
+                                                    const uint32_t* __restrict__ dir, int dir_bits, int idb,
+                                                    uint64_t* __restrict__ pairs, int64_t cap,
+                                                    unsigned long long* __restrict__ ctr) {
+	__shared__ uint32_t s_tmp[MATCH_NT / WAVE + 1];
 	__shared__ unsigned long long s_base;
-	const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+	const int64_t r = (int64_t)blockIdx.x * MATCH_NT + threadIdx.x;
 	uint32_t lo = 0, count = 0;
-	uint64_t key = 0, qid = 0;
+	uint64_t qid = 0;
 	if (r < nq) {
 		const uint64_t q = qrec[r];
-		key = q >> idb;
+		const uint64_t key = q >> idb;
 		qid = q & ((1ull << idb) - 1ull);
-		const uint32_t b = (uint32_t)(q >> (64 - DIR_BITS));
+		const uint32_t b = (uint32_t)(q >> (64 - dir_bits));
 		lo = dir[b];
-		uint32_t hi = dir[b + 1];
+		const uint32_t end = dir[b + 1];
+		uint32_t hi = end;
 		while (lo < hi) {
 			const uint32_t mid = (lo + hi) >> 1;
 			if ((idx[mid] >> idb) < key)
@@ -60,43 +67,14 @@ __global__ __launch_bounds__(256) void k_match(const uint64_t* __restrict__ qrec
 			else
 				hi = mid;
 		}
-		const uint32_t end = dir[b + 1];
 		for (uint32_t m = lo; m < end && (idx[m] >> idb) == key; ++m)
 			++count;
 	}
-	unsigned long long slot = block_append<256>(ctr, count, s_tmp, &s_base);
+	unsigned long long slot = block_append<MATCH_NT>(ctr, count, s_tmp, &s_base);
 	const uint64_t mask = (1ull << idb) - 1ull;
 	for (uint32_t m = lo; m < lo + count; ++m, ++slot)
 		if ((int64_t)slot < cap)
 			pairs[slot] = (qid << idb) | (idx[m] & mask);
-}
-
-// one lane per sorted candidate (query row, index row): duplicates skipped,
-// the rest verified between the query string and the index string
-template <bool WIDE>
-__global__ __launch_bounds__(256) void k_verify_cross(const uint64_t* __restrict__ pairs, int64_t n,
-                                                      int idb, const uint8_t* __restrict__ qchars,
-                                                      const uint32_t* __restrict__ qoffs,
-                                                      const uint8_t* __restrict__ ichars,
-                                                      const uint32_t* __restrict__ ioffs, int distance,
-                                                      int measure, uint32_t* __restrict__ flags,
-                                                      int8_t* __restrict__ dist) {
-	const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (q >= n)
-		return;
-	const uint64_t p = pairs[q];
-	uint32_t keep = 0;
-	int8_t dv = 0;
-	if (q == 0 || pairs[q - 1] != p) {
-		const uint32_t a = (uint32_t)(p >> idb), b = (uint32_t)(p & ((1ull << idb) - 1ull));
-		const uint32_t sa = qoffs[a], sb = ioffs[b];
-		const int d = pair_distance<WIDE>(qchars + sa, (int)(qoffs[a + 1] - sa), ichars + sb,
-		                            (int)(ioffs[b + 1] - sb), distance, measure);
-		keep = d <= distance ? 1u : 0u;
-		dv = (int8_t)d;
-	}
-	flags[q] = keep;
-	dist[q] = dv;
 }
 
 }  // namespace xtn
