@@ -195,6 +195,23 @@ int xtn_server_wait(xtn_server* s, uint64_t ticket, xtn_callback callback, void*
 void xtn_server_close(xtn_server* s);
 const char* xtn_server_last_error(xtn_server* s);
 
+/* ---- search without device allocation (no reference counterpart).
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. The same
+ * search as xtn_run_nn_resident on the caller's device buffers: d_chars and
+ * d_offsets[n + 1] as xtn_set_sequences takes them (residues are not
+ * validated here), d_pairs / d_dist of out_cap entries receive the sorted
+ * (x, y, distance) triples, *out_len their count; stream is a hipStream_t
+ * (NULL: the default stream), waited on before the call returns. Every
+ * workspace is a slice of one static device array of 52 MB per GPU, held
+ * until the process ends; the call makes no allocation and creates no stream
+ * or event. Limits: n <= 65536 sequences, at most 2^20 candidate pairs and
+ * 2^20 output pairs; beyond them XTN_ERR_CAPACITY. Calls on one GPU share its
+ * array: they are serialised inside the library, and calls from several
+ * processes use separate arrays. */
+int xtn_nn_noalloc(const char* d_chars, const uint32_t* d_offsets, int32_t n, int32_t distance, int32_t measure,
+                   xtn_int2* d_pairs, int8_t* d_dist, int64_t out_cap, int64_t* out_len, void* stream);
+
 /* profile == 1 only: launches, summed HIP-event duration (ms) and summed
  * algorithmic HBM bytes of one kernel over the runs since the last reset. */
 int xtn_kernel_profile(xtn_handle* h, const char* kernel, int64_t* launches, double* ms,

@@ -111,6 +111,7 @@ struct xtn_handle {
 	bool graph_want_set = false;
 	int64_t graph_launches = 0;  // kernels the graph holds
 	bool capturing = false;
+	bool no_alloc = false;  // xtn_nn_noalloc: every buffer is a slice of the static scratch
 	int64_t alloc_gen = 0;  // bumped by every device allocation of the handle
 	void* pin = nullptr;  // pinned upload staging of a server worker
 	size_t pin_bytes = 0;
@@ -175,6 +176,11 @@ int ensure(xtn_handle* h, DBuf& b, size_t bytes) {
 		return XTN_OK;
 	if (h && h->capturing)  // the caller abandons the capture and issues the run eagerly
 		return fail(h, XTN_ERR_STATE, "allocation during graph capture");
+	if (h && h->no_alloc)  // buffers are fixed slices of the static scratch
+		return fail(h, XTN_ERR_CAPACITY,
+		            "the search exceeds the no-allocation scratch (a buffer of %zu bytes is needed where %zu are "
+		            "set aside)",
+		            bytes, b.cap);
 	if (b.p) {
 		HIP_SYNC(h, h->stream);
 		HIP_TRY(h, hipFree(b.p));
@@ -1062,7 +1068,9 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		g.rs = rs;
 		// more than 8 bucket bits: an exact pass over the top 8, then the
 		// region scatter (fixed regions of bucket_cap slots in recs)
-		g.region = g.B > 8;
+		// (the no-allocation form keeps to the sorted layout: its scratch holds
+		// two record buffers, not the regions)
+		g.region = g.B > 8 && !h->no_alloc;
 		TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)cap_recs));
 		TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)cap_recs));
 		if (g.region)
@@ -1881,6 +1889,147 @@ const char* xtn_status_string(int status) {
 }
 
 const char* xtn_version(void) { return "2.0.0-mi355x"; }
+
+}  // extern "C"
+
+// ============================================== no-allocation search ==
+//
+// xtn_nn_noalloc runs the search of xtn_run_nn_resident on the caller's
+// device buffers, with every workspace a fixed slice of one file-scope
+// device array (`scratch`) instead of handle allocations: no hipMalloc,
+// hipFree, stream or event creation. Sizes beyond the slices are rejected.
+// The array of a device is shared by every call on it: calls are serialised
+// per device by a host mutex, and every kernel clears what it needs cleared.
+
+namespace {
+
+constexpr int64_t NA_MAX_SEQS = 1 << 16;     // sequences
+constexpr int64_t NA_MAX_RECORDS = 1 << 20;  // deletion records (sum of the variant counts)
+constexpr int64_t NA_MAX_PAIRS = 1 << 20;    // candidate pairs of the search, and output pairs
+constexpr size_t NA_SCRATCH_BYTES = (size_t)52 << 20;
+
+__device__ __align__(256) uint8_t scratch[NA_SCRATCH_BYTES];
+
+struct NoAllocDevice {
+	std::mutex mu;
+	uint8_t* ptr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
+	bool ready = false;
+};
+
+NoAllocDevice g_noalloc[64];
+
+// carves the handle's buffers out of the scratch; false when the layout does
+// not fit (a build-time inconsistency)
+bool carve_scratch(xtn_handle* h, uint8_t* base) {
+	size_t off = 0;
+	auto take = [&](DBuf& b, size_t bytes) {
+		bytes = (bytes + 255) & ~(size_t)255;
+		b.p = base + off;
+		b.cap = bytes;
+		off += bytes;
+	};
+	const int64_t rtiles = NA_MAX_RECORDS / RS_TILE + 1;
+	const int64_t sort_tiles = NA_MAX_PAIRS / RS_TILE + 1, sel_tiles = NA_MAX_PAIRS / SL_TILE + 1;
+	take(h->pk, 16 * NA_MAX_SEQS);
+	take(h->voff, 8 * (NA_MAX_SEQS + 1));
+	take(h->tseq, 4 * (NA_MAX_RECORDS / GEN_TILE + 2));
+	take(h->recs, 8 * NA_MAX_RECORDS);
+	take(h->recs_alt, 8 * NA_MAX_RECORDS);
+	take(h->hist, 4 * RS_BINS * rtiles);
+	take(h->scan_tmp, 64 << 10);
+	take(h->runs_small, sizeof(Run) * (NA_MAX_RECORDS / 2 + 1));
+	take(h->runs_big, sizeof(Run) * (NA_MAX_RECORDS / (SMALL_RUN_MAX + 1) + 1));
+	take(h->big_pre, 8 * (NA_MAX_RECORDS / (SMALL_RUN_MAX + 1) + 2));
+	take(h->bstart, 4 * (65536 + 1));
+	take(h->bcount, 4 * 65536);
+	take(h->ovf, 4 * 65536);
+	take(h->ctl, 4 * CTL_WORDS);
+	take(h->ctrs, sizeof(CtlBlock));
+	take(h->part_hist, 8 * 4096);
+	take(h->pairs, 8 * NA_MAX_PAIRS);
+	take(h->pairs_alt, 8 * NA_MAX_PAIRS);
+	take(h->out_pairs, 8 * NA_MAX_PAIRS);
+	take(h->out_dist, NA_MAX_PAIRS);
+	take(h->tail_state, 4 * (4 * sort_tiles * RS_BINS + 2 * sel_tiles));  // 2 * idb <= 32 bits: 4 passes
+	take(h->pos, 4 * (NA_MAX_PAIRS / SEL_TILE + NA_MAX_PAIRS / SEL_NT + 4));
+	take(h->dist, NA_MAX_PAIRS);
+	return off <= NA_SCRATCH_BYTES;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xtn_nn_noalloc(const char* d_chars, const uint32_t* d_offsets, int32_t n, int32_t distance, int32_t measure,
+                   xtn_int2* d_pairs, int8_t* d_dist, int64_t out_cap, int64_t* out_len, void* stream) {
+	if (!out_len || n < 0 || (n > 0 && (!d_chars || !d_offsets)) || out_cap < 0 ||
+	    (out_cap > 0 && (!d_pairs || !d_dist)))
+		return XTN_ERR_ARG;
+	*out_len = 0;
+	if (distance < 1 || distance > 127 || (measure != XTN_LEVENSHTEIN && measure != XTN_HAMMING))
+		return XTN_ERR_ARG;
+	if (n > NA_MAX_SEQS)
+		return XTN_ERR_CAPACITY;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+		(void)hipGetLastError();
+		return XTN_ERR_HIP;
+	}
+	NoAllocDevice& nd = g_noalloc[dev];
+	std::lock_guard<std::mutex> lk(nd.mu);
+	if (!nd.ready) {
+		// once per device, with the device current: this device's address of
+		// the array (the host-side name is no device address)
+		void* p = nullptr;
+		if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+			(void)hipGetLastError();
+			return XTN_ERR_HIP;
+		}
+		nd.ptr = static_cast<uint8_t*>(p);
+		nd.ready = true;
+	}
+	if (n < 2)
+		return XTN_OK;
+	xtn_handle h;
+	xtn_config_init(&h.cfg);
+	h.cfg.distance = distance;
+	h.cfg.measure = measure;
+	h.cfg.device = dev;
+	h.cfg.no_graph = 1;
+	h.cfg.max_records_per_chunk = NA_MAX_RECORDS;
+	h.cfg.max_pairs_per_round = NA_MAX_PAIRS;
+	h.stream = static_cast<hipStream_t>(stream);
+	h.no_alloc = true;
+	h.free_valid = true;  // no free-memory query: the slices are the budget
+	h.free_cache = NA_SCRATCH_BYTES;
+	h.last_pairs = NA_MAX_PAIRS;  // the candidate capacity is the whole slice
+	int ncu = 0;
+	if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+		h.n_cu = ncu;
+	if (!carve_scratch(&h, nd.ptr))
+		return XTN_ERR_CAPACITY;
+	h.chars.p = const_cast<char*>(d_chars);
+	h.chars.cap = ~(size_t)0 >> 1;
+	h.offs.p = const_cast<uint32_t*>(d_offsets);
+	h.offs.cap = ~(size_t)0 >> 1;
+	h.n = n;
+	int rc = nn_pipeline(&h, n, nullptr);
+	if (rc != XTN_OK)
+		return rc;
+	if (h.out_len > out_cap)
+		return XTN_ERR_CAPACITY;
+	if (h.out_len > 0) {
+		if (hipMemcpyAsync(d_pairs, h.out_pairs.p, sizeof(xtn_int2) * (size_t)h.out_len, hipMemcpyDeviceToDevice,
+		                   h.stream) != hipSuccess ||
+		    hipMemcpyAsync(d_dist, h.out_dist.p, (size_t)h.out_len, hipMemcpyDeviceToDevice, h.stream) != hipSuccess ||
+		    hipStreamSynchronize(h.stream) != hipSuccess) {
+			(void)hipGetLastError();
+			return XTN_ERR_HIP;
+		}
+	}
+	*out_len = h.out_len;
+	return XTN_OK;
+}
 
 }  // extern "C"
 

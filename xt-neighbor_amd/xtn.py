@@ -37,6 +37,7 @@ EXPORTED = [
     "xtn_kernel_profile", "xtn_kernel_profile_reset", "xtn_last_error", "xtn_status_string",
     "xtn_version", "xtn_perform_sets", "xtn_server_config_init", "xtn_server_open",
     "xtn_server_submit", "xtn_server_wait", "xtn_server_close", "xtn_server_last_error",
+    "xtn_nn_noalloc",
 ]
 
 
@@ -161,6 +162,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.xtn_server_close.restype = None
     lib.xtn_server_last_error.argtypes = [vp]
     lib.xtn_server_last_error.restype = ctypes.c_char_p
+    lib.xtn_nn_noalloc.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp,
+                                   ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
+    lib.xtn_nn_noalloc.restype = ctypes.c_int
     for name in ("xtn_open", "xtn_set_config", "xtn_set_sequences", "xtn_perform",
                  "xtn_run_nn_resident", "xtn_copy_nn_output", "xtn_get_stats", "xtn_perform_sets",
                  "xtn_kernel_profile", "xtn_kernel_profile_reset"):
