@@ -268,7 +268,8 @@ def end_to_end(chars, offsets, d, measure):
     return out
 
 
-def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int, e2e: bool) -> dict:
+def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int, e2e: bool,
+                tune: bool = True) -> dict:
     """Repertoire overlap on the batched-repertoire workload: n_reps
     repertoires of log-uniform size, Zipf(2) frequencies capped at 1e5, 10%
     of sequences shared or 1-edit-shared across repertoires, Levenshtein d=1.
@@ -284,6 +285,7 @@ def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int
            "measure": "leven"}
     with xtn.Engine(distance=1) as eng:
         eng.set_sequences(chars=chars, offsets=offsets)
+        out["tuning"] = autotuned(eng, tune)
         res = eng.overlap(freq, rep)
         times = []
         for _ in range(steps):
@@ -317,6 +319,25 @@ def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int
     return out
 
 
+TUNE_CACHE = None
+
+
+def autotuned(eng, enabled: bool) -> dict:
+    """the engine's autotuner run on its current workload (cache in a fresh
+    temporary file, so the choice is measured in this run): chosen parameters,
+    their run time and the defaults' run time"""
+    global TUNE_CACHE
+    if not enabled:
+        return {"enabled": False, **eng.tuning()}
+    if TUNE_CACHE is None:
+        import tempfile
+        TUNE_CACHE = os.path.join(tempfile.mkdtemp(prefix="xtn_tune_"), "tuning.json")
+    t = eng.autotune(TUNE_CACHE, runs=5)
+    return {"enabled": True, "bucket_capacity": t["bucket_capacity"], "bucket_target": t["bucket_target"],
+            "sort_threads": t["sort_threads"], "ms_tuned": t["ms"], "ms_default": t["ms_default"],
+            "candidates": t["candidates"]}
+
+
 def timed_runs(eng, steps):
     """ms per nn_resident run (median of `steps` after one warm-up) and the stats"""
     eng.nn_resident()
@@ -330,7 +351,7 @@ def timed_runs(eng, steps):
     return statistics.median(times) * 1e3, eng.stats()
 
 
-def chunked_leg(steps: int = 5) -> dict:
+def chunked_leg(steps: int = 5, tune: bool = True) -> dict:
     """5M synthetic CDR3 (lengths ~N(15,2.5) in [6,30], 25% planted 1-edit
     relatives), d=1, Levenshtein and Hamming: one key-space chunk against the
     key space forced into chunks of at most 20M records (each chunk
@@ -342,6 +363,8 @@ def chunked_leg(steps: int = 5) -> dict:
     with xtn.Engine(distance=1) as eng:
         eng.set_sequences(chars=chars, offsets=offsets)
         for measure in ("leven", "hamming"):
+            eng.configure(measure=measure, max_records_per_chunk=0)
+            out[f"{measure}_tuning"] = autotuned(eng, tune)
             for label, chunk in (("one_chunk", 0), ("chunks", 20_000_000)):
                 eng.configure(measure=measure, max_records_per_chunk=chunk)
                 ms, st = timed_runs(eng, steps)
@@ -409,6 +432,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-overlap", action="store_true")
     ap.add_argument("--no-chunked", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default engine parameters")
     ap.add_argument("--no-distances", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--served-index", type=int, default=10_000_000)
@@ -434,6 +458,7 @@ def main():
 
     eng = xtn.Engine(distance=d, measure=measure, profile=False)
     eng.set_sequences(chars=chars, offsets=offsets)
+    tuning = autotuned(eng, not args.no_autotune)
     for _ in range(args.warmup):
         eng.nn_resident()
 
@@ -505,6 +530,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, dom)},
         "kernels": per_kernel,
+        "tuning": tuning,
         "reference_published": {"seqs_per_s": 183500, "note": "1M Emerson CDR3, d=2, 5.449 s end to "
                                 "end incl. process start and parsing, GPU unrecorded "
                                 "(data/gpu_benchmark.csv:42); not the same measurement"},
@@ -523,13 +549,13 @@ def main():
         result["served"] = served(args.served_index, args.served_queries, 5)
         progress("served done")
     if not args.no_chunked:
-        result["chunked_5M_d1"] = chunked_leg()
+        result["chunked_5M_d1"] = chunked_leg(tune=not args.no_autotune)
         progress("chunked 5M done")
     if not args.no_distances:
         result["distances_100k"] = distance_leg()
         progress("distance scaling done")
     if not args.no_overlap:
-        result["overlap"] = overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e)
+        result["overlap"] = overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e, not args.no_autotune)
         progress("overlap done")
     result["copy_probe"] = copy_probe()
     progress("copy probe done")

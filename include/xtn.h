@@ -163,6 +163,34 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 
 int xtn_get_stats(xtn_handle* h, xtn_stats* out);
 
+/* ---- engine parameters and their autotuner (no reference counterpart: the
+ * reference's sizes are constants, xtn_inner.cu:14, xtn.cu:36-46) */
+typedef struct {
+	int32_t bucket_capacity;  /* records one workgroup groups in LDS: 2048, 3072 (default) or
+	                             4096 (62 KB of LDS, two workgroups per CU) */
+	int32_t bucket_target;    /* mean records per bucket (sets the bucket bits); default 2200 */
+	int32_t sort_threads;     /* threads of a candidate-sort tile: 256 or 512 (default) */
+	int32_t from_cache;       /* xtn_autotune: 1 when read from the cache file */
+	double ms;                /* xtn_autotune: run time with these parameters */
+	double ms_default;        /* xtn_autotune: run time with the defaults */
+	int32_t candidates;       /* xtn_autotune: settings timed */
+	int32_t reserved;
+} xtn_tuning;
+
+int xtn_get_tuning(xtn_handle* h, xtn_tuning* out);
+/* XTN_ERR_ARG for values outside the lists above (bucket_target must lie in
+ * [bucket_capacity / 4, bucket_capacity * 0.9]) */
+int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t);
+/* Chooses the parameters for the sequences set on the handle and its
+ * distance and measure: each candidate setting is run `runs` times (after
+ * two warm-up runs) as xtn_run_nn_resident, its output checked against the
+ * default setting's, and the fastest kept (the default unless another is
+ * faster by more than 1%). With cache_path, the choice is read from that JSON
+ * file when an entry for the workload (sequence count, distance, measure,
+ * variants per sequence) is there, else written to it. The handle keeps the
+ * choice; *out (nullable) receives it. */
+int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning* out);
+
 /* ---- served form (no reference counterpart: query sets against a resident index).
  * A server keeps the sorted deletion index of n index sequences resident in
  * HBM and answers query sets against it: every (query row, index row) pair

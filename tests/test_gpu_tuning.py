@@ -1,0 +1,72 @@
+"""The autotuner and the engine parameters it chooses, on the GPU: every
+bucket capacity (2048, 3072, 4096 records in LDS), bucket target and sort
+tile width gives exactly the oracle's output; xtn_autotune keeps a setting
+it has checked against the defaults and reads it back from its cache."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import xtn
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def triples(p, d):
+    return np.concatenate([p.astype(np.int64), d.astype(np.int64)[:, None]], axis=1)
+
+
+@pytest.mark.parametrize("cap,target,nt", [(2048, 1500, 256), (4096, 3400, 512), (4096, 1100, 256),
+                                           (3072, 2600, 256)])
+def test_settings_match_oracle(cap, target, nt):
+    chars, offsets = xtn.synth_nn(51, 150_000, planted=0.25, max_edits=2)
+    want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        eng.set_tuning(bucket_capacity=cap, bucket_target=target, sort_threads=nt)
+        for _ in range(3):  # eager, captured, replayed
+            n = eng.nn_resident()
+            assert np.array_equal(triples(*eng.copy_nn_output(n)), want)
+        assert eng.tuning()["bucket_capacity"] == cap
+
+
+def test_invalid_settings_rejected():
+    with xtn.Engine(distance=1) as eng:
+        for kw in (dict(bucket_capacity=1000), dict(sort_threads=128), dict(bucket_capacity=2048, bucket_target=2000),
+                   dict(bucket_capacity=4096, bucket_target=500)):
+            with pytest.raises(xtn.XtnError) as e:
+                eng.set_tuning(**kw)
+            assert e.value.status == 3
+
+
+def test_autotune_and_cache(tmp_path):
+    chars, offsets = xtn.synth_nn(52, 120_000, planted=0.25, max_edits=2)
+    want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    cache = str(tmp_path / "tuning.json")
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        t = eng.autotune(cache, runs=2)
+        assert t["from_cache"] == 0 and t["candidates"] == 18
+        assert t["ms"] <= t["ms_default"]
+        n = eng.nn_resident()
+        assert np.array_equal(triples(*eng.copy_nn_output(n)), want)
+    with open(cache) as f:
+        doc = json.load(f)
+    assert len(doc["entries"]) == 1 and doc["entries"][0]["bucket_capacity"] == t["bucket_capacity"]
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        t2 = eng.autotune(cache, runs=2)
+        assert t2["from_cache"] == 1
+        assert (t2["bucket_capacity"], t2["bucket_target"], t2["sort_threads"]) == \
+            (t["bucket_capacity"], t["bucket_target"], t["sort_threads"])
+        n = eng.nn_resident()
+        assert np.array_equal(triples(*eng.copy_nn_output(n)), want)
+        # another workload gets an entry of its own
+        eng.configure(distance=1)
+        eng.autotune(cache, runs=1)
+    with open(cache) as f:
+        assert len(json.load(f)["entries"]) == 2

@@ -1036,8 +1036,8 @@ __device__ __forceinline__ void bucket_range(const BucketArgs& a, int b, uint32_
 constexpr int BK_BIN_BITS = 12;
 constexpr int BK_BINS = 1 << BK_BIN_BITS;
 constexpr int BK_NT = 512;
-constexpr int BK_CAP = 3072;             // largest bucket grouped in LDS
-constexpr int BK_PER = BK_CAP / BK_NT;   // records per thread
+constexpr int BK_CAP = 3072;             // largest bucket grouped in LDS (default)
+constexpr int BK_CAP_MAX = 4096;         // largest instantiated: 62 KB of LDS, two workgroups per CU
 constexpr uint16_t BK_NONE = 0xFFFF;
 
 constexpr int BK_STAGE = 376;  // pairs staged in LDS between flushes
@@ -1057,7 +1057,8 @@ constexpr int BK_GROUP = 3;     // record slots per walk pass: a wave lists at m
 // bucket count; nothing waits on another workgroup. 49 KB of LDS: three
 // workgroups (24 waves) per CU (at 53.8 KB only two fit on MI355X).
 // records of bucket b into registers (nothing for an overflowing bucket)
-__device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t (&k)[BK_PER],
+template <int PER>
+__device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t (&k)[PER],
                                             uint32_t* s, uint32_t* e) {
 	*s = 0;
 	*e = 0;
@@ -1065,18 +1066,21 @@ __device__ __forceinline__ void bucket_load(const BucketArgs& a, int b, uint64_t
 		bucket_range(a, b, s, e);
 	const int c = (int)(*e - *s);
 #pragma unroll
-	for (int i = 0; i < BK_PER; ++i) {
+	for (int i = 0; i < PER; ++i) {
 		const int j = i * BK_NT + (int)threadIdx.x;
 		k[i] = (c <= a.cap && j < c) ? __builtin_nontemporal_load(&a.rec[*s + j]) : 0ull;  // read once
 	}
 }
 
 // RANGED: the round covers only part of the smaller ids (the id test is
-// compiled out of the common single-round case)
-template <bool RANGED>
+// compiled out of the common single-round case). CAP: the largest bucket
+// the workgroup holds (2048, 3072 or 4096 records; a.cap <= CAP).
+template <bool RANGED, int CAP>
 __global__ __launch_bounds__(BK_NT) void k_bucket(BucketArgs a) {
-	__shared__ uint64_t s_keys[BK_CAP];
-	__shared__ uint16_t s_next[BK_CAP];
+	constexpr int BK_PER = CAP / BK_NT;  // records per thread
+	static_assert(CAP % (2 * BK_NT) == 0, "two links per register");
+	__shared__ uint64_t s_keys[CAP];
+	__shared__ uint16_t s_next[CAP];
 	__shared__ uint32_t s_head[BK_BINS];  // (iteration << 16) | record of the bin's last push
 	__shared__ uint64_t s_stage[BK_STAGE];
 	__shared__ uint16_t s_walk[BK_WAVES][BK_GROUP * WAVE];  // each wave's records that have a predecessor

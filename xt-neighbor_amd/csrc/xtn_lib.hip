@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -102,7 +103,7 @@ struct xtn_handle {
 	struct GraphSig {
 		int64_t nseq, row_base, V, pcap, alloc_gen;
 		const void *seq_map, *set_of, *chars, *offs;
-		int32_t d, measure, bucket_cap, pad;
+		int32_t d, measure, bucket_cap, target, sort_nt, pad;
 		bool operator==(const GraphSig& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 	};
 	hipGraph_t graph = nullptr;
@@ -110,6 +111,11 @@ struct xtn_handle {
 	GraphSig graph_sig{}, graph_want{};
 	bool graph_want_set = false;
 	int64_t graph_launches = 0;  // kernels the graph holds
+	// engine parameters the autotuner chooses (xtn_set_tuning / xtn_autotune)
+	int tune_cap = 3072;      // bucket kernel instantiation: 2048, 3072 or 4096 records
+	int tune_target = 2200;   // mean records per bucket
+	int tune_sort_nt = 512;   // threads of a pair-sort tile: 256 or 512
+	double tune_ms = 0, tune_ms_default = 0;
 	bool capturing = false;
 	bool no_alloc = false;  // xtn_nn_noalloc: every buffer is a slice of the static scratch
 	int64_t alloc_gen = 0;  // bumped by every device allocation of the handle
@@ -486,7 +492,6 @@ double now_ms() {
 
 // ----------------------------------------------------------- NN pipeline --
 
-constexpr int64_t BUCKET_TARGET = 2200;  // mean records per LDS bucket (<= BK_CAP fit)
 constexpr int KEY_BITS = 32;             // hash bits that order records into runs
 
 // device control block: the counters of the candidate search, then the
@@ -654,8 +659,11 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 	const unsigned sort_grid = (unsigned)std::max<int64_t>(tp.sort_tiles, 1);
 	for (int p = 0; p < tp.passes; ++p) {
 		ProfScope ps(h, "sort_pairs", 16.0 * cand);
-		k_onesweep<OS_NT><<<sort_grid, OS_NT, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc,
-		                                             state + (int64_t)p * tp.sort_tiles * RS_BINS, h->spin_limit);
+		uint32_t* st = state + (int64_t)p * tp.sort_tiles * RS_BINS;
+		if (h->tune_sort_nt == 256)
+			k_onesweep<256><<<sort_grid, 256, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+		else
+			k_onesweep<512><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
 		HIP_LAUNCH_CHECK(h);
 		std::swap(a, b);
 	}
@@ -740,6 +748,7 @@ int tail_rts(xtn_handle* h, int64_t n_pairs, int idb, const uint32_t* seq_map, u
 struct Grouping {
 	int64_t nrec = 0;
 	int64_t nseq = 0;  // ids of the run: a round over [0, nseq) is not ranged
+	int kernel_cap = BK_CAP;  // the bucket kernel instantiation (2048, 3072, 4096)
 	int B = 0, SB = 0, nb = 1;
 	int hi = 64, rs = 0;  // the chunk's hash bits end at hi; runs are equal rec >> rs
 	bool region = false;  // fixed bucket regions (more than 8 bucket bits)
@@ -827,12 +836,18 @@ int issue_buckets(xtn_handle* h, Grouping& g, uint32_t lo, uint32_t hi, int64_t 
 	const bool ranged = lo > 0 || (int64_t)hi < g.nseq;
 	StageScope sg(h, "pairs");
 	ProfScope ps(h, "bucket", (double)g.nrec * 8.0);
-	// persistent workgroups, three per CU
-	const unsigned grid = (unsigned)std::min<int64_t>(g.nb, (int64_t)h->n_cu * 3);
-	if (ranged)
-		k_bucket<true><<<grid, BK_NT, 0, h->stream>>>(ba);
+	// persistent workgroups, as many per CU as the LDS of the instantiation
+	// leaves room for (160 KB per CU)
+	const int kcap = g.kernel_cap;
+	const int per_cu = kcap <= 3072 ? 3 : 2;
+	const unsigned grid = (unsigned)std::min<int64_t>(g.nb, (int64_t)h->n_cu * per_cu);
+	hipStream_t s = h->stream;
+	if (kcap <= 2048)
+		ranged ? k_bucket<true, 2048><<<grid, BK_NT, 0, s>>>(ba) : k_bucket<false, 2048><<<grid, BK_NT, 0, s>>>(ba);
+	else if (kcap <= 3072)
+		ranged ? k_bucket<true, 3072><<<grid, BK_NT, 0, s>>>(ba) : k_bucket<false, 3072><<<grid, BK_NT, 0, s>>>(ba);
 	else
-		k_bucket<false><<<grid, BK_NT, 0, h->stream>>>(ba);
+		ranged ? k_bucket<true, 4096><<<grid, BK_NT, 0, s>>>(ba) : k_bucket<false, 4096><<<grid, BK_NT, 0, s>>>(ba);
 	HIP_LAUNCH_CHECK(h);
 	return XTN_OK;
 }
@@ -960,7 +975,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 
 	// chunk plan: the records of one chunk (plus the sort buffer and the
 	// bucket regions, ~40 B per record in all) must fit the HBM budget, and
-	// its buckets must stay near BUCKET_TARGET records
+	// its buckets must stay near the target record count
 	if (!h->free_valid) {
 		size_t total_b = 0;
 		HIP_TRY(h, hipMemGetInfo(&h->free_cache, &total_b));
@@ -970,7 +985,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	int64_t budget = h->cfg.max_records_per_chunk;
 	if (budget <= 0) {
 		budget = (int64_t)((double)free_b * 0.6 / 40.0);
-		budget = std::min<int64_t>(budget, (int64_t)65536 * BUCKET_TARGET);
+		budget = std::min<int64_t>(budget, (int64_t)65536 * h->tune_target);
 	}
 	budget = std::max<int64_t>(budget, 1 << 16);
 	int pbits = 0;
@@ -1048,18 +1063,19 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	TRY(ensure(h, h->bcount, sizeof(uint32_t) * 65536));
 	TRY(ensure(h, h->ctl, sizeof(uint32_t) * CTL_WORDS));
 	TRY(ensure(h, h->ovf, sizeof(uint32_t) * 65536));
-	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, BK_CAP) : BK_CAP;
+	const int bucket_cap = h->cfg.bucket_capacity > 0 ? std::min(h->cfg.bucket_capacity, h->tune_cap) : h->tune_cap;
 
 	// the grouping plan of a chunk of nrec records
 	auto plan_chunk = [&](int64_t nrec, Grouping& g) -> int {
 		g = Grouping{};
 		g.nrec = nrec;
 		g.nseq = nseq;
+		g.kernel_cap = h->tune_cap;
 		// bucket bits: the B hash bits right below the chunk bits, B such that
-		// buckets hold about BUCKET_TARGET records; the sort orders the
+		// buckets hold about tune_target records; the sort orders the
 		// records by the next multiple of 8 bits
 		int B = 0;
-		while (B < 16 && (nrec >> B) > BUCKET_TARGET)
+		while (B < 16 && (nrec >> B) > h->tune_target)
 			++B;
 		g.B = std::min(B, K);
 		g.SB = (g.B + 7) / 8 * 8;
@@ -1184,6 +1200,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				sig.d = d;
 				sig.measure = h->cfg.measure;
 				sig.bucket_cap = bucket_cap;
+				sig.target = h->tune_target;
+				sig.sort_nt = h->tune_sort_nt;
 			}
 			bool via_graph = false;
 			if (graph_ok && h->graph_exec && sig == h->graph_sig) {
@@ -1842,6 +1860,32 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 	return XTN_OK;
 }
 
+int xtn_get_tuning(xtn_handle* h, xtn_tuning* out) {
+	if (!h || !out)
+		return XTN_ERR_ARG;
+	memset(out, 0, sizeof(*out));
+	out->bucket_capacity = h->tune_cap;
+	out->bucket_target = h->tune_target;
+	out->sort_threads = h->tune_sort_nt;
+	out->ms = h->tune_ms;
+	out->ms_default = h->tune_ms_default;
+	return XTN_OK;
+}
+
+int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t) {
+	if (!h || !t)
+		return XTN_ERR_ARG;
+	const int c = t->bucket_capacity;
+	if ((c != 2048 && c != 3072 && c != 4096) || (t->sort_threads != 256 && t->sort_threads != 512) ||
+	    t->bucket_target < c / 4 || t->bucket_target > c * 9 / 10)
+		return fail(h, XTN_ERR_ARG, "invalid tuning (capacity %d, target %d, sort threads %d)", c, t->bucket_target,
+		            t->sort_threads);
+	h->tune_cap = c;
+	h->tune_target = t->bucket_target;
+	h->tune_sort_nt = t->sort_threads;
+	return XTN_OK;
+}
+
 int xtn_get_stats(xtn_handle* h, xtn_stats* out) {
 	if (!h || !out)
 		return XTN_ERR_ARG;
@@ -1886,6 +1930,215 @@ const char* xtn_status_string(int status) {
 	case XTN_ERR_STATE: return "invalid call order";
 	default: return "unknown status";
 	}
+}
+
+// ---- autotuner: the workload's signature, the cache file, the search --
+
+static std::string tune_signature(xtn_handle* h) {
+	uint64_t per[19];
+	for (int L = 0; L <= 18; ++L)
+		per[L] = host_variant_count((uint32_t)L, h->cfg.distance);
+	uint64_t V = 0;
+	for (uint8_t L : h->prefix_len)
+		V += per[L];
+	const double n = (double)std::max<int32_t>(h->n, 1);
+	char buf[128];
+	// sequence count in quarter octaves, variants per sequence to two decimals
+	snprintf(buf, sizeof(buf), "n%.2f_d%d_m%d_v%.2f", std::round(4.0 * std::log2(n)) / 4.0, h->cfg.distance,
+	         h->cfg.measure, (double)V / n);
+	return buf;
+}
+
+// the cache holds one JSON object per line inside {"xtn_tuning": 1, "entries": [ ... ]}
+static bool tune_cache_read(const char* path, const std::string& sig, xtn_tuning* t) {
+	FILE* f = fopen(path, "r");
+	if (!f)
+		return false;
+	char line[1024];
+	bool found = false;
+	const std::string key = "\"workload\": \"" + sig + "\"";
+	while (!found && fgets(line, sizeof(line), f)) {
+		if (!strstr(line, key.c_str()))
+			continue;
+		auto num = [&](const char* name, double* v) {
+			const char* p = strstr(line, name);
+			return p && sscanf(p + strlen(name), " : %lf", v) == 1;
+		};
+		double c, tg, nt, ms, msd;
+		if (num("\"bucket_capacity\"", &c) && num("\"bucket_target\"", &tg) && num("\"sort_threads\"", &nt) &&
+		    num("\"ms\"", &ms) && num("\"ms_default\"", &msd)) {
+			t->bucket_capacity = (int32_t)c;
+			t->bucket_target = (int32_t)tg;
+			t->sort_threads = (int32_t)nt;
+			t->ms = ms;
+			t->ms_default = msd;
+			found = true;
+		}
+	}
+	fclose(f);
+	return found;
+}
+
+static bool tune_cache_write(const char* path, const std::string& sig, const xtn_tuning& t) {
+	std::vector<std::string> keep;
+	if (FILE* f = fopen(path, "r")) {
+		char line[1024];
+		const std::string key = "\"workload\": \"" + sig + "\"";
+		while (fgets(line, sizeof(line), f))
+			if (strstr(line, "\"workload\"") && !strstr(line, key.c_str())) {
+				std::string l(line);
+				while (!l.empty() && (l.back() == '\n' || l.back() == ',' || l.back() == ' '))
+					l.pop_back();
+				keep.push_back(l);
+			}
+		fclose(f);
+	}
+	char buf[512];
+	snprintf(buf, sizeof(buf),
+	         "  {\"workload\": \"%s\", \"bucket_capacity\": %d, \"bucket_target\": %d, \"sort_threads\": %d, "
+	         "\"ms\": %.4f, \"ms_default\": %.4f}",
+	         sig.c_str(), t.bucket_capacity, t.bucket_target, t.sort_threads, t.ms, t.ms_default);
+	keep.push_back(buf);
+	const std::string tmp = std::string(path) + ".tmp";
+	FILE* f = fopen(tmp.c_str(), "w");
+	if (!f)
+		return false;
+	fprintf(f, "{\"xtn_tuning\": 1, \"entries\": [\n");
+	for (size_t i = 0; i < keep.size(); ++i)
+		fprintf(f, "%s%s\n", keep[i].c_str(), i + 1 < keep.size() ? "," : "");
+	fprintf(f, "]}\n");
+	const bool ok = fclose(f) == 0;
+	return ok && rename(tmp.c_str(), path) == 0;
+}
+
+// FNV-1a over the output triples of the last run, copied to the host
+static int output_digest(xtn_handle* h, uint64_t* out) {
+	std::vector<xtn_int2> p;
+	std::vector<int8_t> d;
+	const int64_t n = h->out_len;
+	p.resize((size_t)std::max<int64_t>(n, 1));
+	d.resize((size_t)std::max<int64_t>(n, 1));
+	if (n > 0) {
+		HIP_TRY(h, hipMemcpyAsync(p.data(), h->out_pairs.p, sizeof(xtn_int2) * (size_t)n, hipMemcpyDeviceToHost,
+		                          h->stream));
+		HIP_TRY(h, hipMemcpyAsync(d.data(), h->out_dist.p, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+		HIP_SYNC(h, h->stream);
+	}
+	uint64_t x = 1469598103934665603ull;
+	auto mix = [&](const void* b, size_t len) {
+		const unsigned char* c = (const unsigned char*)b;
+		for (size_t i = 0; i < len; ++i)
+			x = (x ^ c[i]) * 1099511628211ull;
+	};
+	if (n > 0) {
+		mix(p.data(), sizeof(xtn_int2) * (size_t)n);
+		mix(d.data(), (size_t)n);
+	}
+	mix(&n, sizeof(n));
+	*out = x;
+	return XTN_OK;
+}
+
+int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning* out) {
+	if (!h || runs < 1)
+		return XTN_ERR_ARG;
+	if (!h->offs.p)
+		return fail(h, XTN_ERR_STATE, "no sequences set");
+	HIP_TRY(h, hipSetDevice(h->cfg.device));
+	const std::string sig = tune_signature(h);
+	xtn_tuning best{};
+	if (cache_path && tune_cache_read(cache_path, sig, &best) && xtn_set_tuning(h, &best) == XTN_OK) {
+		best.from_cache = 1;
+		h->tune_ms = best.ms;
+		h->tune_ms_default = best.ms_default;
+		if (out)
+			*out = best;
+		return XTN_OK;
+	}
+	// one setting: two warm-up runs (an eager run and its graph capture),
+	// then the median host time of `runs` runs, each ending in the readback
+	int32_t runs_now = runs;
+	auto time_setting = [&](int cap, int target, int nt, double* ms, uint64_t* digest) -> int {
+		h->tune_cap = cap;
+		h->tune_target = target;
+		h->tune_sort_nt = nt;
+		std::vector<double> t;
+		for (int r = 0; r < runs_now + 2; ++r) {
+			memset(&h->stats, 0, sizeof(h->stats));
+			TRY(nn_pipeline(h, h->n, nullptr));
+			if (r >= 2)
+				t.push_back(h->stats.ms_total);
+		}
+		std::sort(t.begin(), t.end());
+		*ms = t[t.size() / 2];
+		return output_digest(h, digest);
+	};
+	const int def_cap = 3072, def_target = 2200, def_nt = 512;
+	double ms_def = 0;
+	uint64_t dig_def = 0;
+	TRY(time_setting(def_cap, def_target, def_nt, &ms_def, &dig_def));
+	best.bucket_capacity = def_cap;
+	best.bucket_target = def_target;
+	best.sort_threads = def_nt;
+	best.ms = ms_def;
+	int tried = 1;
+	// bucket targets at about 0.6, 0.72 and 0.85 of each capacity (the
+	// default's 2200 among them)
+	const int caps[3] = {3072, 4096, 2048};
+	const int targets[3][3] = {{1840, 2200, 2600}, {2450, 2950, 3480}, {1230, 1470, 1740}};
+	const int nts[2] = {512, 256};
+	for (int ci = 0; ci < 3; ++ci)
+		for (int target : targets[ci])
+			for (int nt : nts) {
+				const int c = caps[ci];
+				if (c == def_cap && target == def_target && nt == def_nt)
+					continue;
+				double ms = 0;
+				uint64_t dig = 0;
+				TRY(time_setting(c, target, nt, &ms, &dig));
+				++tried;
+				if (dig != dig_def)
+					return fail(h, XTN_ERR_HIP, "autotune: capacity %d, target %d, sort threads %d changed the output",
+					            c, target, nt);
+				if (ms < best.ms) {
+					best.bucket_capacity = c;
+					best.bucket_target = target;
+					best.sort_threads = nt;
+					best.ms = ms;
+				}
+			}
+	// the best against the defaults once more, interleaved, with twice the
+	// runs: a setting is kept only when it still wins (the first timings of a
+	// search carry the order they were taken in)
+	if (best.bucket_capacity != def_cap || best.bucket_target != def_target || best.sort_threads != def_nt) {
+		const int c = best.bucket_capacity, tg = best.bucket_target, nt = best.sort_threads;
+		double a1 = 0, b1 = 0, a2 = 0, b2 = 0;
+		uint64_t dig = 0;
+		runs_now = 2 * runs;
+		TRY(time_setting(def_cap, def_target, def_nt, &a1, &dig));
+		TRY(time_setting(c, tg, nt, &b1, &dig));
+		TRY(time_setting(def_cap, def_target, def_nt, &a2, &dig));
+		TRY(time_setting(c, tg, nt, &b2, &dig));
+		runs_now = runs;
+		ms_def = std::min(a1, a2);
+		best.ms = std::min(b1, b2);
+	}
+	if (best.ms > 0.99 * ms_def) {  // keep the defaults unless clearly beaten
+		best.bucket_capacity = def_cap;
+		best.bucket_target = def_target;
+		best.sort_threads = def_nt;
+		best.ms = ms_def;
+	}
+	best.ms_default = ms_def;
+	best.candidates = tried;
+	TRY(xtn_set_tuning(h, &best));
+	h->tune_ms = best.ms;
+	h->tune_ms_default = ms_def;
+	if (cache_path && !tune_cache_write(cache_path, sig, best))
+		return fail(h, XTN_ERR_ARG, "autotune: cannot write the cache file %s", cache_path);
+	if (out)
+		*out = best;
+	return XTN_OK;
 }
 
 const char* xtn_version(void) { return "2.0.0-mi355x"; }

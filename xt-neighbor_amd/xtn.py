@@ -37,7 +37,7 @@ EXPORTED = [
     "xtn_kernel_profile", "xtn_kernel_profile_reset", "xtn_last_error", "xtn_status_string",
     "xtn_version", "xtn_perform_sets", "xtn_server_config_init", "xtn_server_open",
     "xtn_server_submit", "xtn_server_wait", "xtn_server_close", "xtn_server_last_error",
-    "xtn_nn_noalloc",
+    "xtn_nn_noalloc", "xtn_get_tuning", "xtn_set_tuning", "xtn_autotune",
 ]
 
 
@@ -90,6 +90,13 @@ class Stats(ctypes.Structure):
                 ("ms_output", ctypes.c_double), ("n_rounds", ctypes.c_int64),
                 ("tail_checked", ctypes.c_int64), ("n_launches", ctypes.c_int64),
                 ("n_syncs", ctypes.c_int64), ("graph_replays", ctypes.c_int64)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("bucket_capacity", ctypes.c_int32), ("bucket_target", ctypes.c_int32),
+                ("sort_threads", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("ms", ctypes.c_double), ("ms_default", ctypes.c_double),
+                ("candidates", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Output))
@@ -165,6 +172,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.xtn_nn_noalloc.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp,
                                    ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
     lib.xtn_nn_noalloc.restype = ctypes.c_int
+    lib.xtn_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.xtn_get_tuning.restype = ctypes.c_int
+    lib.xtn_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    lib.xtn_set_tuning.restype = ctypes.c_int
+    lib.xtn_autotune.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(Tuning)]
+    lib.xtn_autotune.restype = ctypes.c_int
     for name in ("xtn_open", "xtn_set_config", "xtn_set_sequences", "xtn_perform",
                  "xtn_run_nn_resident", "xtn_copy_nn_output", "xtn_get_stats", "xtn_perform_sets",
                  "xtn_kernel_profile", "xtn_kernel_profile_reset"):
@@ -474,6 +487,23 @@ class Engine:
         c = SET_CALLBACK(cb) if collect else ctypes.cast(None, SET_CALLBACK)
         self._check(self.lib.xtn_perform_sets(self.h, _u32p(so), len(so) - 1, 1 if batched else 0, c, None))
         return res
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(self.lib.xtn_get_tuning(self.h, ctypes.byref(t)))
+        return {name: getattr(t, name) for name, _ in Tuning._fields_}
+
+    def set_tuning(self, bucket_capacity: int = 3072, bucket_target: int = 2200, sort_threads: int = 512):
+        t = Tuning(bucket_capacity=bucket_capacity, bucket_target=bucket_target, sort_threads=sort_threads)
+        self._check(self.lib.xtn_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, cache_path: str | None = None, runs: int = 5) -> dict:
+        """chooses and keeps the engine parameters for the current sequences
+        (xtn_autotune); returns them with the measured times"""
+        t = Tuning()
+        self._check(self.lib.xtn_autotune(self.h, cache_path.encode() if cache_path else None, runs,
+                                          ctypes.byref(t)))
+        return {name: getattr(t, name) for name, _ in Tuning._fields_}
 
     def stats(self) -> dict:
         s = Stats()
