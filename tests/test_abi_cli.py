@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import xtn
@@ -154,3 +155,55 @@ def test_cli_airr_parse_errors(tmp_path):
     bad_num = write(tmp_path, "a3.tsv", hdr + "x\t1\tCAAA\ny\t0\tCADA\n")
     r = run_cli(["-a", "-i", bad_num, "-n", "2", "-I", info, "-N", "1"])
     assert r.stderr == b"Error: invalid number\nError line: 1\nError: line parsing error\nError line: 1\n"
+
+
+def _dump(tmp_path, args, serial):
+    env = dict(os.environ)
+    dump = str(tmp_path / ("serial.bin" if serial else "parallel.bin"))
+    env["XTN_CLI_PARSE_DUMP"] = dump
+    env.pop("XTN_CLI_SERIAL_PARSE", None)
+    if serial:
+        env["XTN_CLI_SERIAL_PARSE"] = "1"
+    r = subprocess.run([xtn.CLI_PATH] + args, capture_output=True, env=env)
+    data = open(dump, "rb").read() if os.path.exists(dump) and r.returncode == 0 else b""
+    if os.path.exists(dump):
+        os.remove(dump)
+    return r.returncode, r.stdout, r.stderr, data
+
+
+@pytest.mark.parametrize("case", ["plain", "blank_lines", "two_columns", "no_final_newline", "long_line",
+                                  "bad_char_late", "freq_on_next_line", "count_mismatch"])
+def test_parallel_parser_equals_serial(tmp_path, case):
+    """the threaded CSV parser (files over 1 MB) gives the serial parser's
+    store, output and messages byte for byte, including the cases it hands
+    back to the serial parser"""
+    rng = np.random.default_rng(sum(map(ord, case)))
+    seqs = ["".join(rng.choice(list("ACDEFGHIKLMNPQRSTVWY"), int(rng.integers(6, 22)))) for _ in range(80000)]
+    two = case == "two_columns" or case == "freq_on_next_line"
+    lines = [s + (f",{int(rng.integers(1, 99999))}" if two else "") for s in seqs]
+    n = len(lines)
+    if case == "blank_lines":
+        for k in range(0, n, 997):
+            lines.insert(k, "" if k % 2 else " ")
+    if case == "long_line":
+        lines[40000] = "C" * 600
+    if case == "bad_char_late":
+        lines[79000] = "CASSXF"
+    if case == "freq_on_next_line":
+        lines[50000] = seqs[50000] + ",  "
+    body = "cdr3" + (",freq" if two else "") + "\n" + "\n".join(lines)
+    if case != "no_final_newline":
+        body += "\n"
+    path = tmp_path / "in.csv"
+    path.write_text(body)
+    count = n + (1 if case == "count_mismatch" else 0)
+    args = ["-i", str(path), "-n", str(count), "-d", "1"]
+    if two:
+        info = tmp_path / "info.csv"
+        info.write_text(f"size\n{n // 2}\n{n - n // 2}\n")
+        args += ["-I", str(info), "-N", "2"]
+    ser = _dump(tmp_path, args, True)
+    par = _dump(tmp_path, args, False)
+    assert par == ser
+    if case in ("plain", "blank_lines", "two_columns", "no_final_newline"):
+        assert ser[0] == 0 and len(ser[3]) > 4 * n

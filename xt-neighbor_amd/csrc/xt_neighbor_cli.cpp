@@ -5,12 +5,19 @@
 // xtneighbor_streaming/src/cli.cu:10-277 and airr.cu:9-151. Input is parsed
 // and validated on the host before the GPU is touched, so every parse error
 // path runs without a device.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
 #include <climits>
 #include <clocale>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "xtn.h"
@@ -199,6 +206,191 @@ int parse_input(const char* path, SeqStore* seqs, std::vector<xtn_seq_info>* inf
 	// sequence; the reference then runs on uninitialised rows, here it is
 	// reported as a length mismatch
 	if (inputCount != len || seqs->size() != len)
+		return print_err("input length doesn't match with the actual");
+	return SUCCESS;
+}
+
+// parse_input on host threads, for files where it is exactly the serial
+// parse: every line shorter than the 499 characters one fgets call takes (a
+// longer line would be split into several), no NUL byte, no error, no more
+// sequences than announced, and (two columns) a frequency field that strtol
+// reads within its line. Returns SUCCESS or ERROR as parse_input would, or
+// FALLBACK when the serial parser must decide (it then prints the reference's
+// exact message and line number). The file is mapped; each thread takes a
+// range of whole lines, counts and checks them, then writes its residues and
+// offsets straight into the store at its prefix.
+constexpr int FALLBACK = -1;
+
+struct Residues {
+	bool ok[256];
+	Residues() {
+		for (int c = 0; c < 256; ++c)
+			ok[c] = char_encode((char)c) != -1;
+	}
+};
+
+// strtol(p, nullptr, 10) for a field that ends at `end` (a line's newline):
+// false when strtol could read past `end` (a field of white space or a bare
+// sign, after which strtol would skip the newline into the next line)
+bool field_long(const char* p, const char* end, long* out) {
+	while (p < end && (*p == ' ' || *p == '\t' || *p == '\v' || *p == '\f' || *p == '\r'))
+		++p;
+	bool neg = false;
+	if (p < end && (*p == '+' || *p == '-'))
+		neg = *p++ == '-';
+	if (p >= end || *p < '0' || *p > '9')
+		return false;
+	long v = 0;
+	bool over = false;
+	for (; p < end && *p >= '0' && *p <= '9'; ++p) {
+		const int d = *p - '0';
+		if (v > (LONG_MAX - d) / 10)
+			over = true;
+		else
+			v = v * 10 + d;
+	}
+	*out = over ? (neg ? LONG_MIN : LONG_MAX) : (neg ? -v : v);
+	return true;
+}
+
+int parse_input_parallel(const char* path, SeqStore* seqs, std::vector<xtn_seq_info>* info, int len,
+                         bool doubleCol) {
+	static const Residues R;
+	const int fd = open(path, O_RDONLY);
+	if (fd < 0)
+		return FALLBACK;
+	struct stat sb;
+	if (fstat(fd, &sb) != 0 || sb.st_size <= 0) {
+		close(fd);
+		return FALLBACK;
+	}
+	const size_t size = (size_t)sb.st_size;
+	void* m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+	close(fd);
+	if (m == MAP_FAILED)
+		return FALLBACK;
+	struct Unmap {
+		void* m;
+		size_t n;
+		~Unmap() { munmap(m, n); }
+	} unmap{m, size};
+	const char* b = static_cast<const char*>(m);
+	if (memchr(b, '\0', size))
+		return FALLBACK;
+	// the header: the first fgets line
+	const char* nl = (const char*)memchr(b, '\n', size);
+	const size_t body = nl ? (size_t)(nl - b) + 1 : size;
+	if ((nl ? (size_t)(nl - b) : size) >= 499)
+		return FALLBACK;
+	int T = (int)std::thread::hardware_concurrency();
+	T = T < 1 ? 1 : (T > 16 ? 16 : T);
+	if (size - body < ((size_t)1 << 20))
+		T = 1;
+	std::vector<size_t> cut(T + 1, size);  // chunk starts at line starts
+	cut[0] = body;
+	for (int t = 1; t < T; ++t) {
+		size_t p = body + (size - body) * (size_t)t / (size_t)T;
+		if (p < cut[t - 1])
+			p = cut[t - 1];
+		const char* q = p < size ? (const char*)memchr(b + p, '\n', size - p) : nullptr;
+		cut[t] = q ? (size_t)(q - b) + 1 : size;
+	}
+	struct Part {
+		size_t seqs = 0, chars = 0;
+		int counted = 0;
+		bool bad = false;
+	};
+	std::vector<Part> parts(T);
+	// walks the lines of chunk t; with `out`, writes them from the part's bases
+	auto scan = [&](int t, bool write, size_t seq_base, size_t char_base, int count_base) {
+		Part& pt = parts[t];
+		size_t a = cut[t];
+		const size_t end = cut[t + 1];
+		size_t ns = 0, nc = 0;
+		int cnt = 0;
+		while (a < end) {
+			const char* e = (const char*)memchr(b + a, '\n', end - a);
+			const size_t stop = e ? (size_t)(e - b) : end;  // the line is [a, stop)
+			const size_t ll = stop - a;
+			const bool has_nl = e != nullptr;
+			const size_t next = has_nl ? stop + 1 : end;
+			if (!write && ll >= 499) {
+				pt.bad = true;
+				return;
+			}
+			if (!has_nl && ll == 0)
+				break;  // nothing after the last newline
+			if (has_nl && (ll == 0 || (ll == 1 && b[a] == ' '))) {  // "\n" or " \n"
+				a = next;
+				continue;
+			}
+			size_t i = a;
+			while (i < stop && b[i] != ',' && b[i] != '\t') {
+				if (!write && !R.ok[(unsigned char)b[i]]) {
+					pt.bad = true;
+					return;
+				}
+				++i;
+			}
+			const size_t fl = i - a;
+			if (fl > 0) {
+				if (write) {
+					memcpy(seqs->chars.data() + char_base + nc, b + a, fl);
+					seqs->offsets[seq_base + ns + 1] = (uint32_t)(char_base + nc + fl);
+				}
+				++ns;
+				nc += fl;
+			}
+			if (doubleCol) {
+				const char* comma = (const char*)memchr(b + a, ',', ll);
+				long temp = 0;
+				if (!comma || !field_long(comma + 1, b + stop, &temp) || temp == 0 || temp > INT_MAX ||
+				    temp < INT_MIN) {
+					pt.bad = true;
+					return;
+				}
+				const int row = count_base + cnt;
+				if (write && row < (int)info->size())
+					(*info)[row].frequency = (int32_t)temp;
+			}
+			++cnt;
+			a = next;
+		}
+		if (!write) {
+			pt.seqs = ns;
+			pt.chars = nc;
+			pt.counted = cnt;
+		}
+	};
+	auto run = [&](auto fn) {
+		if (T == 1) {
+			fn(0);
+			return;
+		}
+		std::vector<std::thread> th;
+		for (int t = 0; t < T; ++t)
+			th.emplace_back(fn, t);
+		for (auto& x : th)
+			x.join();
+	};
+	run([&](int t) { scan(t, false, 0, 0, 0); });
+	std::vector<size_t> sbase(T + 1, 0), cbase(T + 1, 0);
+	std::vector<int> nbase(T + 1, 0);
+	for (int t = 0; t < T; ++t) {
+		if (parts[t].bad)
+			return FALLBACK;
+		sbase[t + 1] = sbase[t] + parts[t].seqs;
+		cbase[t + 1] = cbase[t] + parts[t].chars;
+		nbase[t + 1] = nbase[t] + parts[t].counted;
+	}
+	const size_t stored = sbase[T];
+	if (stored > (size_t)seqs->maxSize || cbase[T] > (size_t)UINT32_MAX)
+		return FALLBACK;
+	seqs->chars.resize(cbase[T]);
+	seqs->offsets.resize(stored + 1);
+	seqs->offsets[0] = 0;
+	run([&](int t) { scan(t, true, sbase[t], cbase[t], nbase[t]); });
+	if (nbase[T] != len || (int)stored != len)
 		return print_err("input length doesn't match with the actual");
 	return SUCCESS;
 }
@@ -404,12 +596,62 @@ void overlap_handler(void* user, const xtn_output* o) {
 
 }  // namespace
 
+// stage timings on stderr when XTN_CLI_TIMING is set (the bench reads them)
+struct Timing {
+	bool on = getenv("XTN_CLI_TIMING") != nullptr;
+	std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now(), last = t0;
+	std::string line;
+	void mark(const char* stage) {
+		if (!on)
+			return;
+		const auto now = std::chrono::steady_clock::now();
+		char buf[64];
+		snprintf(buf, sizeof(buf), " %s_ms=%.3f", stage, std::chrono::duration<double, std::milli>(now - last).count());
+		line += buf;
+		last = now;
+	}
+	~Timing() {
+		if (on)
+			fprintf(stderr, "xtn_cli_times%s total_ms=%.3f\n", line.c_str(),
+			        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+	}
+};
+
 int main(int argc, char** argv) {
+	Timing timing;
 	Args args;
 	setlocale(LC_ALL, "");  // cli.cu:234
 	int rc = parse_args(argc, argv, &args);
 	if (rc != SUCCESS)
 		return rc;
+
+	// the device is opened on a second thread while the input is parsed;
+	// nothing is printed there, and a parse error still reports exactly as
+	// before (the opener is joined and its handle closed)
+	xtn_config cfg;
+	xtn_config_init(&cfg);
+	cfg.distance = args.distance;
+	cfg.measure = args.measure;
+	cfg.verbose = args.verbose;
+	if (const char* e = getenv("XTN_MAX_RECORDS_PER_CHUNK"))
+		cfg.max_records_per_chunk = atoll(e);
+	xtn_handle* h = nullptr;
+	int st_open = XTN_ERR_HIP;
+	std::thread opener([&] { st_open = xtn_open(&h, &cfg); });
+	struct Join {
+		std::thread& t;
+		~Join() {
+			if (t.joinable())
+				t.join();
+		}
+	} join_opener{opener};
+	auto close_engine = [&] {
+		if (opener.joinable())
+			opener.join();
+		if (h)
+			xtn_close(h);
+		h = nullptr;
+	};
 
 	const bool overlapMode = args.infoPath != nullptr;
 	SeqStore seqs;
@@ -422,32 +664,55 @@ int main(int argc, char** argv) {
 	if (overlapMode) {
 		info.assign(args.seqLen > 0 ? (size_t)args.seqLen : 0, xtn_seq_info{0, 0, 0});
 		rc = parse_info(args.infoPath, &info, args.infoLen, args.seqLen);
-		if (rc != SUCCESS)
+		if (rc != SUCCESS) {
+			close_engine();
 			return rc;
+		}
 	}
-	if (args.airr)
+	if (args.airr) {
 		rc = parse_airr_input(args.seqPath, &seqs, &info, args.seqLen, overlapMode);
-	else
-		rc = parse_input(args.seqPath, &seqs, &info, args.seqLen, overlapMode);
-	if (rc != SUCCESS)
+	} else {
+		// XTN_CLI_SERIAL_PARSE: the serial parser only (tests compare the two)
+		rc = getenv("XTN_CLI_SERIAL_PARSE") ? FALLBACK
+		                                     : parse_input_parallel(args.seqPath, &seqs, &info, args.seqLen, overlapMode);
+		if (rc == FALLBACK) {
+			seqs.chars.clear();
+			seqs.offsets.assign(1, 0);
+			rc = parse_input(args.seqPath, &seqs, &info, args.seqLen, overlapMode);
+		}
+	}
+	if (rc != SUCCESS) {
+		close_engine();
 		return rc;
+	}
+	timing.mark("parse");
+	if (const char* dump = getenv("XTN_CLI_PARSE_DUMP")) {
+		// tests: the parsed store (offsets, residues, frequencies) and exit
+		close_engine();
+		FILE* f = fopen(dump, "wb");
+		if (!f)
+			return print_err("dump file opening failed");
+		const uint32_t n = (uint32_t)seqs.size();
+		fwrite(&n, sizeof(n), 1, f);
+		fwrite(seqs.offsets.data(), sizeof(uint32_t), seqs.offsets.size(), f);
+		fwrite(seqs.chars.data(), 1, seqs.chars.size(), f);
+		for (const xtn_seq_info& x : info)
+			fwrite(&x, sizeof(x), 1, f);
+		fclose(f);
+		return SUCCESS;
+	}
 	if (args.verbose)
 		print_args(args);
 
-	xtn_config cfg;
-	xtn_config_init(&cfg);
-	cfg.distance = args.distance;
-	cfg.measure = args.measure;
-	cfg.verbose = args.verbose;
-	if (const char* e = getenv("XTN_MAX_RECORDS_PER_CHUNK"))
-		cfg.max_records_per_chunk = atoll(e);
-	xtn_handle* h = nullptr;
-	int st = xtn_open(&h, &cfg);
+	opener.join();
+	timing.mark("open_wait");
+	int st = st_open;
 	if (st != XTN_OK) {
 		fprintf(stderr, "Error: cannot open the GPU engine (%s)\n", xtn_status_string(st));
 		return ERROR;
 	}
 	st = xtn_set_sequences(h, seqs.chars.data(), seqs.offsets.data(), seqs.size());
+	timing.mark("upload");
 	if (st != XTN_OK) {
 		fprintf(stderr, "Error: %s\n", xtn_last_error(h));
 		xtn_close(h);
@@ -463,6 +728,7 @@ int main(int argc, char** argv) {
 	}
 	st = xtn_perform(h, overlapMode ? info.data() : nullptr, overlapMode ? overlap_handler : nn_handler,
 	                 &sink);
+	timing.mark("perform");
 	if (sink.out)
 		fclose(sink.out);
 	if (st != XTN_OK) {
@@ -471,6 +737,7 @@ int main(int argc, char** argv) {
 		return ERROR;
 	}
 	xtn_close(h);
+	timing.mark("close");
 	if (args.verbose)
 		printf("total output length: %'lu\n", (unsigned long)sink.total);
 	return SUCCESS;
