@@ -169,7 +169,10 @@ typedef struct {
 	int32_t bucket_capacity;  /* records one workgroup groups in LDS: 2048, 3072 (default) or
 	                             4096 (62 KB of LDS, two workgroups per CU) */
 	int32_t bucket_target;    /* mean records per bucket (sets the bucket bits); default 2200 */
-	int32_t sort_threads;     /* threads of a candidate-sort tile: 256 or 512 (default) */
+	int32_t sort_threads;     /* threads of a candidate-sort tile: 256, 512 (default) or 1024 */
+	int32_t sort_tile;        /* keys of a candidate-sort tile: 4096 (default; with 256 or 512
+	                             threads) or 8192 (with 512 or 1024 threads) */
+	int32_t select_threads;   /* threads of a select tile, 8 candidates each: 256 (default) or 512 */
 	int32_t from_cache;       /* xtn_autotune: 1 when read from the cache file */
 	double ms;                /* xtn_autotune: run time with these parameters */
 	double ms_default;        /* xtn_autotune: run time with the defaults */
@@ -184,8 +187,10 @@ int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t);
 /* Chooses the parameters for the sequences set on the handle and its
  * distance and measure: each candidate setting is run `runs` times (after
  * two warm-up runs) as xtn_run_nn_resident, its output checked against the
- * default setting's, and the fastest kept (the default unless another is
- * faster by more than 1%). With cache_path, the choice is read from that JSON
+ * default setting's; the search goes coordinate by coordinate (bucket
+ * capacity and target, then the sort tile, then the select tile) and keeps
+ * the fastest, the default unless another beats it by more than 1% when
+ * both are timed again. With cache_path, the choice is read from that JSON
  * file when an entry for the workload (sequence count, distance, measure,
  * variants per sequence) is there, else written to it. The handle keeps the
  * choice; *out (nullable) receives it. */

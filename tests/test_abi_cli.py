@@ -207,3 +207,39 @@ def test_parallel_parser_equals_serial(tmp_path, case):
     assert par == ser
     if case in ("plain", "blank_lines", "two_columns", "no_final_newline"):
         assert ser[0] == 0 and len(ser[3]) > 4 * n
+
+
+def run_cli1(args, **kw):
+    return subprocess.run([xtn.CLI1_PATH] + args, capture_output=True, **kw)
+
+
+def test_cli1_flags_and_errors(tmp_path):
+    """the non-streaming xtneighbor command line (xtneighbor/src/cli.cu):
+    version, help, argument and input errors, all before the device"""
+    r = run_cli1(["-v"])
+    assert r.returncode == 2 and r.stdout == b"1.0.0\n"
+    r = run_cli1(["-h"])
+    assert r.returncode == 2 and r.stdout.startswith(b"xt_neighbor\n\t description: perform xt_neighbor")
+    assert b"-c or --check-output" in r.stdout
+    cases = [
+        (["-n", "3"], "missing path for seq1"),
+        (["-p", "x"], "missing length for seq1"),
+        (["-p", "x", "-n", "3", "-d", "0"], "distance must be a valid number larger than 0"),
+        (["-p", "x", "-n", "abc"], "invalid input length"),
+        (["-p", "x", "-n", "3", "-m", "leven"], "unknown option"),
+        (["-p", str(tmp_path / "missing.txt"), "-n", "3"], "file reading failed"),
+    ]
+    for args, msg in cases:
+        r = run_cli1(args)
+        assert r.returncode == 1 and r.stderr == f"Error: {msg}\n".encode(), args
+    r = run_cli1(["-p", os.path.join(GOLDEN, "sample_input3.txt"), "-n", "3"])
+    assert r.returncode == 1
+    assert r.stderr == (b"Error: parsing error at line: 3 (only upper-cased amino acids with max length of 18 "
+                        b"are allowed)\n")
+    r = run_cli1(["-p", os.path.join(GOLDEN, "sample_input1.txt"), "-n", "5"])
+    assert r.returncode == 1 and r.stderr == b"Error: input length doesn't match with the actual\n"
+    # a 49-character line is read in two fgets calls, the second "\n" skipped
+    p = tmp_path / "long.txt"
+    p.write_text("C" * 49 + "\nCASS\n")
+    r = run_cli1(["-p", str(p), "-n", "3"])
+    assert r.returncode == 1 and r.stderr == b"Error: input length doesn't match with the actual\n"

@@ -252,3 +252,32 @@ def test_cli_end_to_end(tmp_path):
     assert r.returncode == 0
     assert r.stdout.startswith(b"XTNArgs{\n\tdistance: 1\n")
     assert r.stdout.endswith(b"total output length: 5\n")
+
+
+def test_cli1_outputs_and_check(tmp_path):
+    """the non-streaming xtneighbor command line on the reference's sample
+    inputs and a random set (lines longer than 18 residues are cut there):
+    the output file against the oracle's brute force over the 18-residue
+    forms, and -c's own brute-force check"""
+    env = dict(os.environ, LC_ALL="C")
+    rng = np.random.default_rng(17)
+    rnd = tmp_path / "rnd.txt"
+    rows = random_seqs(rng, 3000, 8, 26, "ACDEFGHIK")
+    rnd.write_text("".join(s + "\n" for s in rows))
+    cases = [(os.path.join(GOLDEN, "sample_input1.txt"), 4), (os.path.join(GOLDEN, "sample_input2.txt"), 3),
+             (str(rnd), 3000)]
+    for path, n in cases:
+        lines = [l for l in open(path).read().split("\n") if l not in ("", " ")]
+        cut = [l[:18] for l in lines]
+        chars, offsets = xtn.encode(cut)
+        for d in (1, 2):
+            out = tmp_path / "o.txt"
+            r = subprocess.run([xtn.CLI1_PATH, "-p", path, "-n", str(n), "-d", str(d), "-o", str(out), "-c", "-V"],
+                               capture_output=True, env=env, timeout=300)
+            assert r.returncode == 0, r.stderr
+            assert b"output is verrified to be correct\n" in r.stdout
+            want = oracle.nn_brute(chars, offsets, d, 0)
+            text = "".join(f"{a} {b} {c}\n" for a, b, c in want.tolist())
+            assert out.read_text() == text
+            assert r.stdout.startswith(b"XTNArgs{\n\tdistance: ")
+            assert r.stdout.endswith(f"Success! Number of triplet: {len(want)}\n".encode())

@@ -103,7 +103,7 @@ struct xtn_handle {
 	struct GraphSig {
 		int64_t nseq, row_base, V, pcap, alloc_gen;
 		const void *seq_map, *set_of, *chars, *offs;
-		int32_t d, measure, bucket_cap, target, sort_nt, pad;
+		int32_t d, measure, bucket_cap, target, sort_nt, sort_tile, select_nt, pad;
 		bool operator==(const GraphSig& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 	};
 	hipGraph_t graph = nullptr;
@@ -114,13 +114,16 @@ struct xtn_handle {
 	// engine parameters the autotuner chooses (xtn_set_tuning / xtn_autotune)
 	int tune_cap = 3072;      // bucket kernel instantiation: 2048, 3072 or 4096 records
 	int tune_target = 2200;   // mean records per bucket
-	int tune_sort_nt = 512;   // threads of a pair-sort tile: 256 or 512
+	int tune_sort_nt = 512;   // threads of a pair-sort tile: 256, 512 or 1024
+	int tune_sort_tile = 4096;  // keys of a pair-sort tile: 4096 or 8192
+	int tune_select_nt = 256;   // threads of a select tile (8 candidates each): 256 or 512
 	double tune_ms = 0, tune_ms_default = 0;
 	bool capturing = false;
 	bool no_alloc = false;  // xtn_nn_noalloc: every buffer is a slice of the static scratch
 	int64_t alloc_gen = 0;  // bumped by every device allocation of the handle
-	void* pin = nullptr;  // pinned upload staging of a server worker
+	void* pin = nullptr;  // pinned upload staging (xtn_set_sequences, a server worker's queries)
 	size_t pin_bytes = 0;
+	hipEvent_t pin_ev[2] = {nullptr, nullptr};
 	double serve_ratio = 0;  // a server worker's candidates per query record so far
 	bool tail_safe = false;  // a look-back wait hit its limit once: host-checked tail from then on
 	uint32_t spin_limit = LB_SPIN_LIMIT;  // XTN_LOOKBACK_SPIN_LIMIT overrides (tests of the fallback)
@@ -621,8 +624,9 @@ struct TailPlan {
 int tail_prepare(xtn_handle* h, int64_t pcap, int idb, int64_t out_keep, TailPlan* tp) {
 	tp->pcap = pcap;
 	tp->passes = (2 * idb + 7) / 8;
-	tp->sort_tiles = (pcap + RS_TILE - 1) / RS_TILE;
-	tp->sel_tiles = (pcap + SL_TILE - 1) / SL_TILE;
+	const int64_t sort_tile = h->tune_sort_tile, sel_tile = (int64_t)h->tune_select_nt * SL_IT;
+	tp->sort_tiles = (pcap + sort_tile - 1) / sort_tile;
+	tp->sel_tiles = (pcap + sel_tile - 1) / sel_tile;
 	TRY(ensure(h, h->pairs_alt, sizeof(uint64_t) * (size_t)pcap));
 	TRY(grow_keep(h, h->out_pairs, sizeof(int2) * (size_t)(out_keep + pcap), sizeof(int2) * (size_t)out_keep));
 	TRY(grow_keep(h, h->out_dist, (size_t)(out_keep + pcap), (size_t)out_keep));
@@ -660,10 +664,15 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 	for (int p = 0; p < tp.passes; ++p) {
 		ProfScope ps(h, "sort_pairs", 16.0 * cand);
 		uint32_t* st = state + (int64_t)p * tp.sort_tiles * RS_BINS;
-		if (h->tune_sort_nt == 256)
-			k_onesweep<256><<<sort_grid, 256, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+		const int nt = h->tune_sort_nt, tile = h->tune_sort_tile;
+		if (nt == 256)
+			k_onesweep<256, 16><<<sort_grid, 256, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+		else if (nt == 512 && tile == 4096)
+			k_onesweep<512, 8><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+		else if (nt == 512)
+			k_onesweep<512, 16><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
 		else
-			k_onesweep<512><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+			k_onesweep<1024, 8><<<sort_grid, 1024, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
 		HIP_LAUNCH_CHECK(h);
 		std::swap(a, b);
 	}
@@ -691,7 +700,7 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 		// kept triples written
 		ProfScope ps(h, "select", 8.0 * cand + 40.0 * (double)h->last_unique + 9.0 * (double)h->last_out);
 		const unsigned g = (unsigned)std::max<int64_t>(1, tp.sel_tiles);
-		select_kernel(h->cfg.distance, h->cfg.measure)<<<g, SL_NT, 0, s>>>(sa);
+		select_kernel(h->cfg.distance, h->cfg.measure, h->tune_select_nt)<<<g, h->tune_select_nt, 0, s>>>(sa);
 		HIP_LAUNCH_CHECK(h);
 	}
 	return XTN_OK;
@@ -1202,6 +1211,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				sig.bucket_cap = bucket_cap;
 				sig.target = h->tune_target;
 				sig.sort_nt = h->tune_sort_nt;
+				sig.sort_tile = h->tune_sort_tile;
+				sig.select_nt = h->tune_select_nt;
 			}
 			bool via_graph = false;
 			if (graph_ok && h->graph_exec && sig == h->graph_sig) {
@@ -1598,6 +1609,9 @@ void xtn_close(xtn_handle* h) {
 		(void)hipEventDestroy(e);
 	if (h->pin)
 		(void)hipHostFree(h->pin);
+	for (hipEvent_t e : h->pin_ev)
+		if (e)
+			(void)hipEventDestroy(e);
 	if (h->graph_exec)
 		(void)hipGraphExecDestroy(h->graph_exec);
 	if (h->graph)
@@ -1617,33 +1631,122 @@ int xtn_set_config(xtn_handle* h, const xtn_config* cfg) {
 	return XTN_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// runs fn(t, lo, hi) over [0, n) split into up to 16 ranges on host threads
+// (one range below `grain` items per thread)
+template <typename Fn>
+void host_parallel(int64_t n, int64_t grain, Fn fn) {
+	int T = (int)std::thread::hardware_concurrency();
+	T = T < 1 ? 1 : (T > 16 ? 16 : T);
+	while (T > 1 && n / T < grain)
+		--T;
+	if (T == 1) {
+		fn(0, (int64_t)0, n);
+		return;
+	}
+	std::vector<std::thread> th;
+	for (int t = 0; t < T; ++t)
+		th.emplace_back(fn, t, n * t / T, n * (t + 1) / T);
+	for (auto& x : th)
+		x.join();
+}
+
+// host bytes through the handle's pinned staging (two halves in turn, so
+// the copy into one half overlaps the transfer of the other) to dst on the
+// handle's stream; waits for the last transfer
+int upload_staged(xtn_handle* h, void* dst, const void* src, size_t bytes) {
+	constexpr size_t HALF = (size_t)8 << 20;
+	if (bytes == 0)
+		return XTN_OK;
+	if (h->pin_bytes < 2 * HALF) {
+		if (h->pin)
+			(void)hipHostFree(h->pin);
+		h->pin = nullptr;
+		h->pin_bytes = 0;
+		if (hipHostMalloc(&h->pin, 2 * HALF, hipHostMallocDefault) != hipSuccess) {
+			(void)hipGetLastError();
+			h->pin = nullptr;
+			// no pinned memory: the runtime's own staging
+			HIP_TRY(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+			HIP_SYNC(h, h->stream);
+			return XTN_OK;
+		}
+		h->pin_bytes = 2 * HALF;
+	}
+	if (!h->pin_ev[0]) {
+		for (int k = 0; k < 2; ++k)
+			if (hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming) != hipSuccess)
+				return fail(h, XTN_ERR_HIP, "staging event creation failed");
+	}
+	char* pin = static_cast<char*>(h->pin);
+	bool used[2] = {false, false};
+	for (size_t off = 0, k = 0; off < bytes; off += HALF, ++k) {
+		const int half = (int)(k & 1);
+		const size_t len = std::min(HALF, bytes - off);
+		if (used[half])
+			HIP_TRY(h, hipEventSynchronize(h->pin_ev[half]));  // this half's previous transfer is done
+		memcpy(pin + half * HALF, static_cast<const char*>(src) + off, len);
+		HIP_TRY(h, hipMemcpyAsync(static_cast<char*>(dst) + off, pin + half * HALF, len, hipMemcpyHostToDevice,
+		                          h->stream));
+		HIP_TRY(h, hipEventRecord(h->pin_ev[half], h->stream));
+		used[half] = true;
+	}
+	HIP_SYNC(h, h->stream);
+	return XTN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets, int32_t n) {
 	if (!h || n < 0 || !offsets || (n > 0 && !chars && offsets[n] > 0))
 		return h ? fail(h, XTN_ERR_ARG, "invalid sequence arguments") : XTN_ERR_ARG;
 	if (offsets[0] != 0)
 		return fail(h, XTN_ERR_ARG, "offsets[0] must be 0");
-	for (int32_t i = 0; i < n; ++i)
-		if (offsets[i + 1] < offsets[i])
-			return fail(h, XTN_ERR_ARG, "offsets must be non-decreasing (row %d)", i);
+	// offsets non-decreasing and the prefix lengths, then every residue one
+	// of the 20 (codec.cu:25-38), on host threads; the first bad row is the
+	// one reported
+	std::vector<uint8_t> plen((size_t)n);
+	std::vector<int64_t> first_bad(16, INT64_MAX);
+	host_parallel(n, 1 << 16, [&](int t, int64_t lo, int64_t hi) {
+		for (int64_t i = lo; i < hi; ++i) {
+			const uint32_t a = offsets[i], b = offsets[i + 1];
+			if (b < a) {
+				first_bad[t] = i;
+				return;
+			}
+			const uint32_t len = b - a;
+			plen[(size_t)i] = (uint8_t)(len < 18 ? len : 18);
+		}
+	});
+	for (int64_t fb : first_bad)
+		if (fb != INT64_MAX) {
+			int64_t row = INT64_MAX;
+			for (int64_t x : first_bad)
+				row = std::min(row, x);
+			return fail(h, XTN_ERR_ARG, "offsets must be non-decreasing (row %lld)", (long long)row);
+		}
 	const uint32_t total = offsets[n];
-	if (!all_residues(chars, total))
-		return fail(h, XTN_ERROR, "invalid char");
+	std::vector<char> bad(16, 0);
+	host_parallel((int64_t)total, (int64_t)1 << 20, [&](int t, int64_t lo, int64_t hi) {
+		bad[t] = all_residues(chars + lo, (size_t)(hi - lo)) ? 0 : 1;
+	});
+	for (char x : bad)
+		if (x)
+			return fail(h, XTN_ERROR, "invalid char");
 	HIP_TRY(h, hipSetDevice(h->cfg.device));
 	TRY(ensure(h, h->chars, (size_t)total + 16));
 	TRY(ensure(h, h->offs, sizeof(uint32_t) * ((size_t)n + 1)));
-	if (total)
-		HIP_TRY(h, hipMemcpyAsync(h->chars.p, chars, total, hipMemcpyHostToDevice, h->stream));
-	HIP_TRY(h, hipMemcpyAsync(h->offs.p, offsets, sizeof(uint32_t) * ((size_t)n + 1),
-	                          hipMemcpyHostToDevice, h->stream));
-	HIP_SYNC(h, h->stream);
+	TRY(upload_staged(h, h->chars.p, chars, total));
+	TRY(upload_staged(h, h->offs.p, offsets, sizeof(uint32_t) * ((size_t)n + 1)));
 	h->n = n;
 	h->n_chars = total;
 	h->out_len = 0;
-	h->prefix_len.resize((size_t)n);
-	for (int32_t i = 0; i < n; ++i) {
-		const uint32_t len = offsets[i + 1] - offsets[i];
-		h->prefix_len[i] = (uint8_t)(len < 18 ? len : 18);
-	}
+	h->prefix_len.swap(plen);
 	h->vprefix.clear();
 	h->vprefix_d = 0;
 	return XTN_OK;
@@ -1867,6 +1970,8 @@ int xtn_get_tuning(xtn_handle* h, xtn_tuning* out) {
 	out->bucket_capacity = h->tune_cap;
 	out->bucket_target = h->tune_target;
 	out->sort_threads = h->tune_sort_nt;
+	out->sort_tile = h->tune_sort_tile;
+	out->select_threads = h->tune_select_nt;
 	out->ms = h->tune_ms;
 	out->ms_default = h->tune_ms_default;
 	return XTN_OK;
@@ -1875,14 +1980,20 @@ int xtn_get_tuning(xtn_handle* h, xtn_tuning* out) {
 int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t) {
 	if (!h || !t)
 		return XTN_ERR_ARG;
-	const int c = t->bucket_capacity;
-	if ((c != 2048 && c != 3072 && c != 4096) || (t->sort_threads != 256 && t->sort_threads != 512) ||
-	    t->bucket_target < c / 4 || t->bucket_target > c * 9 / 10)
-		return fail(h, XTN_ERR_ARG, "invalid tuning (capacity %d, target %d, sort threads %d)", c, t->bucket_target,
-		            t->sort_threads);
+	const int c = t->bucket_capacity, nt = t->sort_threads, tile = t->sort_tile;
+	const bool sort_ok = (nt == 256 && tile == 4096) || (nt == 512 && (tile == 4096 || tile == 8192)) ||
+	                     (nt == 1024 && tile == 8192);
+	if ((c != 2048 && c != 3072 && c != 4096) || !sort_ok ||
+	    (t->select_threads != 256 && t->select_threads != 512) || t->bucket_target < c / 4 ||
+	    t->bucket_target > c * 9 / 10)
+		return fail(h, XTN_ERR_ARG,
+		            "invalid tuning (capacity %d, target %d, sort threads %d, sort tile %d, select threads %d)", c,
+		            t->bucket_target, nt, tile, t->select_threads);
 	h->tune_cap = c;
 	h->tune_target = t->bucket_target;
-	h->tune_sort_nt = t->sort_threads;
+	h->tune_sort_nt = nt;
+	h->tune_sort_tile = tile;
+	h->tune_select_nt = t->select_threads;
 	return XTN_OK;
 }
 
@@ -1964,12 +2075,15 @@ static bool tune_cache_read(const char* path, const std::string& sig, xtn_tuning
 			const char* p = strstr(line, name);
 			return p && sscanf(p + strlen(name), " : %lf", v) == 1;
 		};
-		double c, tg, nt, ms, msd;
+		double c, tg, nt, tl, sn, ms, msd;
 		if (num("\"bucket_capacity\"", &c) && num("\"bucket_target\"", &tg) && num("\"sort_threads\"", &nt) &&
-		    num("\"ms\"", &ms) && num("\"ms_default\"", &msd)) {
+		    num("\"sort_tile\"", &tl) && num("\"select_threads\"", &sn) && num("\"ms\"", &ms) &&
+		    num("\"ms_default\"", &msd)) {
 			t->bucket_capacity = (int32_t)c;
 			t->bucket_target = (int32_t)tg;
 			t->sort_threads = (int32_t)nt;
+			t->sort_tile = (int32_t)tl;
+			t->select_threads = (int32_t)sn;
 			t->ms = ms;
 			t->ms_default = msd;
 			found = true;
@@ -1996,8 +2110,9 @@ static bool tune_cache_write(const char* path, const std::string& sig, const xtn
 	char buf[512];
 	snprintf(buf, sizeof(buf),
 	         "  {\"workload\": \"%s\", \"bucket_capacity\": %d, \"bucket_target\": %d, \"sort_threads\": %d, "
-	         "\"ms\": %.4f, \"ms_default\": %.4f}",
-	         sig.c_str(), t.bucket_capacity, t.bucket_target, t.sort_threads, t.ms, t.ms_default);
+	         "\"sort_tile\": %d, \"select_threads\": %d, \"ms\": %.4f, \"ms_default\": %.4f}",
+	         sig.c_str(), t.bucket_capacity, t.bucket_target, t.sort_threads, t.sort_tile, t.select_threads, t.ms,
+	         t.ms_default);
 	keep.push_back(buf);
 	const std::string tmp = std::string(path) + ".tmp";
 	FILE* f = fopen(tmp.c_str(), "w");
@@ -2056,12 +2171,25 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 		return XTN_OK;
 	}
 	// one setting: two warm-up runs (an eager run and its graph capture),
-	// then the median host time of `runs` runs, each ending in the readback
+	// then the median host time of `runs` runs, each ending in the readback;
+	// its output must be the default setting's
 	int32_t runs_now = runs;
-	auto time_setting = [&](int cap, int target, int nt, double* ms, uint64_t* digest) -> int {
-		h->tune_cap = cap;
-		h->tune_target = target;
-		h->tune_sort_nt = nt;
+	struct Setting {
+		int cap, target, sort_nt, sort_tile, select_nt;
+		bool operator==(const Setting& o) const {
+			return cap == o.cap && target == o.target && sort_nt == o.sort_nt && sort_tile == o.sort_tile &&
+			       select_nt == o.select_nt;
+		}
+	};
+	uint64_t dig_def = 0;
+	bool have_def = false;
+	int tried = 0;
+	auto time_setting = [&](const Setting& st, double* ms) -> int {
+		h->tune_cap = st.cap;
+		h->tune_target = st.target;
+		h->tune_sort_nt = st.sort_nt;
+		h->tune_sort_tile = st.sort_tile;
+		h->tune_select_nt = st.select_nt;
 		std::vector<double> t;
 		for (int r = 0; r < runs_now + 2; ++r) {
 			memset(&h->stats, 0, sizeof(h->stats));
@@ -2071,64 +2199,75 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 		}
 		std::sort(t.begin(), t.end());
 		*ms = t[t.size() / 2];
-		return output_digest(h, digest);
+		uint64_t dig = 0;
+		TRY(output_digest(h, &dig));
+		if (!have_def) {
+			dig_def = dig;
+			have_def = true;
+		} else if (dig != dig_def) {
+			return fail(h, XTN_ERR_HIP,
+			            "autotune: capacity %d, target %d, sort %dx%d, select %d changed the output", st.cap,
+			            st.target, st.sort_nt, st.sort_tile, st.select_nt);
+		}
+		++tried;
+		return XTN_OK;
 	};
-	const int def_cap = 3072, def_target = 2200, def_nt = 512;
+	const Setting def{3072, 2200, 512, 4096, 256};
 	double ms_def = 0;
-	uint64_t dig_def = 0;
-	TRY(time_setting(def_cap, def_target, def_nt, &ms_def, &dig_def));
-	best.bucket_capacity = def_cap;
-	best.bucket_target = def_target;
-	best.sort_threads = def_nt;
-	best.ms = ms_def;
-	int tried = 1;
-	// bucket targets at about 0.6, 0.72 and 0.85 of each capacity (the
-	// default's 2200 among them)
+	TRY(time_setting(def, &ms_def));
+	Setting cur = def;
+	double cur_ms = ms_def;
+	// coordinate descent: bucket capacity and target, then the sort tile,
+	// then the select tile, each step keeping the best so far
 	const int caps[3] = {3072, 4096, 2048};
 	const int targets[3][3] = {{1840, 2200, 2600}, {2450, 2950, 3480}, {1230, 1470, 1740}};
-	const int nts[2] = {512, 256};
+	std::vector<Setting> step1, step2, step3;
 	for (int ci = 0; ci < 3; ++ci)
-		for (int target : targets[ci])
-			for (int nt : nts) {
-				const int c = caps[ci];
-				if (c == def_cap && target == def_target && nt == def_nt)
-					continue;
-				double ms = 0;
-				uint64_t dig = 0;
-				TRY(time_setting(c, target, nt, &ms, &dig));
-				++tried;
-				if (dig != dig_def)
-					return fail(h, XTN_ERR_HIP, "autotune: capacity %d, target %d, sort threads %d changed the output",
-					            c, target, nt);
-				if (ms < best.ms) {
-					best.bucket_capacity = c;
-					best.bucket_target = target;
-					best.sort_threads = nt;
-					best.ms = ms;
-				}
+		for (int tg : targets[ci])
+			step1.push_back({caps[ci], tg, def.sort_nt, def.sort_tile, def.select_nt});
+	for (const std::vector<Setting>* group : {&step1, &step2, &step3}) {
+		std::vector<Setting> cands = *group;
+		if (group == &step2)
+			cands = {{cur.cap, cur.target, 256, 4096, cur.select_nt}, {cur.cap, cur.target, 512, 4096, cur.select_nt},
+			         {cur.cap, cur.target, 512, 8192, cur.select_nt}, {cur.cap, cur.target, 1024, 8192, cur.select_nt}};
+		if (group == &step3)
+			cands = {{cur.cap, cur.target, cur.sort_nt, cur.sort_tile, 256},
+			         {cur.cap, cur.target, cur.sort_nt, cur.sort_tile, 512}};
+		for (const Setting& st : cands) {
+			if (st == cur)
+				continue;
+			double ms = 0;
+			TRY(time_setting(st, &ms));
+			if (ms < cur_ms) {
+				cur = st;
+				cur_ms = ms;
 			}
+		}
+	}
 	// the best against the defaults once more, interleaved, with twice the
-	// runs: a setting is kept only when it still wins (the first timings of a
-	// search carry the order they were taken in)
-	if (best.bucket_capacity != def_cap || best.bucket_target != def_target || best.sort_threads != def_nt) {
-		const int c = best.bucket_capacity, tg = best.bucket_target, nt = best.sort_threads;
+	// runs: a setting is kept only when it still wins by more than 1% (the
+	// first timings of a search carry the order they were taken in)
+	if (!(cur == def)) {
 		double a1 = 0, b1 = 0, a2 = 0, b2 = 0;
-		uint64_t dig = 0;
 		runs_now = 2 * runs;
-		TRY(time_setting(def_cap, def_target, def_nt, &a1, &dig));
-		TRY(time_setting(c, tg, nt, &b1, &dig));
-		TRY(time_setting(def_cap, def_target, def_nt, &a2, &dig));
-		TRY(time_setting(c, tg, nt, &b2, &dig));
+		TRY(time_setting(def, &a1));
+		TRY(time_setting(cur, &b1));
+		TRY(time_setting(def, &a2));
+		TRY(time_setting(cur, &b2));
 		runs_now = runs;
 		ms_def = std::min(a1, a2);
-		best.ms = std::min(b1, b2);
+		cur_ms = std::min(b1, b2);
+		if (cur_ms > 0.99 * ms_def) {
+			cur = def;
+			cur_ms = ms_def;
+		}
 	}
-	if (best.ms > 0.99 * ms_def) {  // keep the defaults unless clearly beaten
-		best.bucket_capacity = def_cap;
-		best.bucket_target = def_target;
-		best.sort_threads = def_nt;
-		best.ms = ms_def;
-	}
+	best.bucket_capacity = cur.cap;
+	best.bucket_target = cur.target;
+	best.sort_threads = cur.sort_nt;
+	best.sort_tile = cur.sort_tile;
+	best.select_threads = cur.select_nt;
+	best.ms = cur_ms;
 	best.ms_default = ms_def;
 	best.candidates = tried;
 	TRY(xtn_set_tuning(h, &best));
@@ -2182,7 +2321,8 @@ bool carve_scratch(xtn_handle* h, uint8_t* base) {
 		off += bytes;
 	};
 	const int64_t rtiles = NA_MAX_RECORDS / RS_TILE + 1;
-	const int64_t sort_tiles = NA_MAX_PAIRS / RS_TILE + 1, sel_tiles = NA_MAX_PAIRS / SL_TILE + 1;
+	// tiles of at least 4096 sort keys and 2048 select candidates
+	const int64_t sort_tiles = NA_MAX_PAIRS / RS_TILE + 1, sel_tiles = NA_MAX_PAIRS / 2048 + 1;
 	take(h->pk, 16 * NA_MAX_SEQS);
 	take(h->voff, 8 * (NA_MAX_SEQS + 1));
 	take(h->tseq, 4 * (NA_MAX_RECORDS / GEN_TILE + 2));

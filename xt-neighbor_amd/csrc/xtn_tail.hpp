@@ -170,7 +170,6 @@ __device__ __forceinline__ int64_t tail_count(const unsigned long long* n_dev, i
 	return n < cap ? n : cap;
 }
 
-constexpr int OS_NT = 512;  // threads of a k_onesweep tile (8 keys each)
 constexpr int TH_NT = 512;
 constexpr int TH_IT = 8;
 
@@ -219,15 +218,17 @@ __global__ __launch_bounds__(TH_NT) void k_tail_hist(const uint64_t* __restrict_
 // keys_in -> keys_out. state: [tiles][RS_BINS] status words of this pass,
 // zeroed. The grid covers the capacity; workgroups past the actual count
 // exit at once.
-template <int NT = RS_NT>
+// NT threads of IT keys each: a tile of NT * IT keys (4096 or 8192; fewer,
+// larger tiles make the look-back chain shorter)
+template <int NT, int IT>
 __global__ __launch_bounds__(NT) void k_onesweep(const uint64_t* __restrict__ keys_in,
                                                  uint64_t* __restrict__ keys_out,
                                                  const unsigned long long* __restrict__ n_dev, int64_t cap,
                                                  int pass, TailCtl* __restrict__ ctl,
                                                  uint32_t* __restrict__ state, uint32_t spin_limit) {
-	constexpr int IT = RS_TILE / NT;
+	constexpr int TILE = NT * IT;
 	constexpr int SEG = IT * WAVE;
-	__shared__ uint64_t s_keys[RS_TILE];
+	__shared__ uint64_t s_keys[TILE];
 	__shared__ uint32_t s_cnt[NT / WAVE][RS_BINS];
 	__shared__ uint32_t s_dpre[RS_BINS + 1];
 	__shared__ uint32_t s_gbase[RS_BINS];
@@ -239,10 +240,10 @@ __global__ __launch_bounds__(NT) void k_onesweep(const uint64_t* __restrict__ ke
 	const int shift = 8 * pass;
 	const int64_t n = tail_count(n_dev, cap);
 	const int64_t tile = blockIdx.x;
-	if (tile * RS_TILE >= n)  // uniform
+	if (tile * TILE >= n)  // uniform
 		return;
-	const int64_t base = tile * RS_TILE;
-	const int tile_n = (int)(n - base < RS_TILE ? n - base : RS_TILE);
+	const int64_t base = tile * TILE;
+	const int tile_n = (int)(n - base < TILE ? n - base : TILE);
 	const int64_t seg = base + (int64_t)w * SEG;
 	const int wrem = tile_n - w * SEG;
 	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? wrem : SEG);
@@ -290,9 +291,7 @@ __global__ __launch_bounds__(NT) void k_onesweep(const uint64_t* __restrict__ ke
 	}
 }
 
-constexpr int SL_NT = 256;
-constexpr int SL_IT = 8;
-constexpr int SL_TILE = SL_NT * SL_IT;  // sorted candidates per select tile
+constexpr int SL_IT = 8;  // sorted candidates per select thread
 
 struct SelectArgs {
 	const uint64_t* keys;  // sorted candidates
@@ -321,13 +320,15 @@ struct SelectArgs {
 // the distance go out at the tile's offset (look-back over the kept counts of
 // the tiles before it) plus their rank in the tile, so the output stays
 // sorted. The last tile writes the unique and kept totals.
-// D: the distance's band fixed at compile time (pair_distance_d).
-template <int D>
-__global__ __launch_bounds__(SL_NT) void k_select(SelectArgs a) {
+// D: the distance's band fixed at compile time (pair_distance_d). NT
+// threads, a tile of NT * SL_IT candidates.
+template <int D, int NT>
+__global__ __launch_bounds__(NT) void k_select(SelectArgs a) {
+	constexpr int SL_TILE = NT * SL_IT;
 	__shared__ uint64_t s_k[SL_TILE];  // the tile as loaded
 	__shared__ uint64_t s_u[SL_TILE];  // its first-of-run pairs, in order
 	__shared__ int8_t s_d[SL_TILE];
-	__shared__ uint32_t s_tmp[SL_NT / WAVE + 1];
+	__shared__ uint32_t s_tmp[NT / WAVE + 1];
 	__shared__ uint32_t s_excl;
 	const int t = threadIdx.x;
 	const int64_t n = tail_count(a.n_dev, a.cap);
@@ -339,8 +340,8 @@ __global__ __launch_bounds__(SL_NT) void k_select(SelectArgs a) {
 	const int64_t base = tile * SL_TILE;
 #pragma unroll
 	for (int r = 0; r < SL_IT; ++r) {
-		const int64_t q = base + (int64_t)r * SL_NT + t;
-		s_k[r * SL_NT + t] = q < n ? a.keys[q] : ~0ull;
+		const int64_t q = base + (int64_t)r * NT + t;
+		s_k[r * NT + t] = q < n ? a.keys[q] : ~0ull;
 	}
 	const uint64_t before = base > 0 ? a.keys[base - 1] : ~0ull;
 	__syncthreads();
@@ -356,13 +357,13 @@ __global__ __launch_bounds__(SL_NT) void k_select(SelectArgs a) {
 		}
 	}
 	uint32_t nu;
-	uint32_t pos = block_excl_scan<SL_NT>(cu, s_tmp, &nu);
+	uint32_t pos = block_excl_scan<NT>(cu, s_tmp, &nu);
 #pragma unroll
 	for (int i = 0; i < SL_IT; ++i)
 		if (uflags & (1u << i))
 			s_u[pos++] = s_k[t * SL_IT + i];
 	__syncthreads();
-	for (uint32_t q = t; q < nu; q += SL_NT) {
+	for (uint32_t q = t; q < nu; q += NT) {
 		const uint64_t p = s_u[q];
 		const uint32_t ia = (uint32_t)(p >> a.idb), ib = (uint32_t)(p & idmask);
 		int8_t dv = -1;
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(SL_NT) void k_select(SelectArgs a) {
 		}
 	}
 	uint32_t nk;
-	const uint32_t kpos = block_excl_scan<SL_NT>(ck, s_tmp, &nk);
+	const uint32_t kpos = block_excl_scan<NT>(ck, s_tmp, &nk);
 	if (t < WAVE) {  // the first wave publishes and looks back
 		uint64_t* st = a.state + tile;
 		const uint64_t mine = ((uint64_t)nu << LB2_SHIFT) | nk;
@@ -426,22 +427,28 @@ __global__ __launch_bounds__(SL_NT) void k_select(SelectArgs a) {
 		}
 }
 
-// the select kernel of a run: the band of its distance (Hamming needs none)
+// the select kernel of a run: the band of its distance (Hamming needs none),
+// NT threads per tile
 typedef void (*SelectFn)(SelectArgs);
-inline SelectFn select_kernel(int distance, int measure) {
+template <int NT>
+inline SelectFn select_kernel_nt(int distance, int measure) {
 	if (measure == 1)
-		return k_select<1>;
+		return k_select<1, NT>;
 	switch (distance) {
-	case 1: return k_select<1>;
-	case 2: return k_select<2>;
-	case 3: return k_select<3>;
-	case 4: return k_select<4>;
-	case 5: return k_select<5>;
-	case 6: return k_select<6>;
-	case 7: return k_select<7>;
-	case 8: return k_select<8>;
-	default: return k_select<0>;
+	case 1: return k_select<1, NT>;
+	case 2: return k_select<2, NT>;
+	case 3: return k_select<3, NT>;
+	case 4: return k_select<4, NT>;
+	case 5: return k_select<5, NT>;
+	case 6: return k_select<6, NT>;
+	case 7: return k_select<7, NT>;
+	case 8: return k_select<8, NT>;
+	default: return k_select<0, NT>;
 	}
+}
+
+inline SelectFn select_kernel(int distance, int measure, int nt) {
+	return nt == 512 ? select_kernel_nt<512>(distance, measure) : select_kernel_nt<256>(distance, measure);
 }
 
 }  // namespace xtn

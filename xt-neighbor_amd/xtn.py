@@ -21,6 +21,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libxtn.so")
 SYNTH_PATH = os.path.join(LIB_DIR, "libxtn_synth.so")
 CLI_PATH = os.path.join(HERE, "bin", "xt_neighbor")
+CLI1_PATH = os.path.join(HERE, "bin", "xtneighbor")  # the first, non-streaming command line
 
 LEVENSHTEIN = 0
 HAMMING = 1
@@ -94,7 +95,8 @@ class Stats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("bucket_capacity", ctypes.c_int32), ("bucket_target", ctypes.c_int32),
-                ("sort_threads", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("sort_threads", ctypes.c_int32), ("sort_tile", ctypes.c_int32),
+                ("select_threads", ctypes.c_int32), ("from_cache", ctypes.c_int32),
                 ("ms", ctypes.c_double), ("ms_default", ctypes.c_double),
                 ("candidates", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
@@ -493,8 +495,10 @@ class Engine:
         self._check(self.lib.xtn_get_tuning(self.h, ctypes.byref(t)))
         return {name: getattr(t, name) for name, _ in Tuning._fields_}
 
-    def set_tuning(self, bucket_capacity: int = 3072, bucket_target: int = 2200, sort_threads: int = 512):
-        t = Tuning(bucket_capacity=bucket_capacity, bucket_target=bucket_target, sort_threads=sort_threads)
+    def set_tuning(self, bucket_capacity: int = 3072, bucket_target: int = 2200, sort_threads: int = 512,
+                   sort_tile: int = 4096, select_threads: int = 256):
+        t = Tuning(bucket_capacity=bucket_capacity, bucket_target=bucket_target, sort_threads=sort_threads,
+                   sort_tile=sort_tile, select_threads=select_threads)
         self._check(self.lib.xtn_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, cache_path: str | None = None, runs: int = 5) -> dict:
