@@ -258,7 +258,8 @@ def end_to_end(chars, offsets, d, measure):
             r = subprocess.run(cmd + extra, capture_output=True, env=env)
             wall = time.perf_counter() - t0
             if r.returncode != 0:
-                raise RuntimeError(r.stderr.decode())
+                raise RuntimeError(f"CLI exit status {r.returncode} ({label}): stderr {r.stderr.decode()[-2000:]!r} "
+                                   f"stdout {r.stdout.decode()[-500:]!r}")
             # the CLI's own stage times (ms): parse (the device opens on a
             # second thread meanwhile), the rest of that open, upload, run and
             # hand-off, close
@@ -320,7 +321,8 @@ def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int
             r = subprocess.run(cmd, capture_output=True)
             wall.append(time.perf_counter() - t0)
             if r.returncode != 0:
-                raise RuntimeError(r.stderr.decode())
+                raise RuntimeError(f"CLI exit status {r.returncode} ({label}): stderr {r.stderr.decode()[-2000:]!r} "
+                                   f"stdout {r.stdout.decode()[-500:]!r}")
         out["cli_no_output_s"] = min(wall)
         out["cli_seqs_per_s"] = n / min(wall)
         for fn in os.listdir(tmp):
@@ -347,7 +349,8 @@ def autotuned(eng, enabled: bool) -> dict:
         TUNE_CACHE = os.path.join(tempfile.mkdtemp(prefix="xtn_tune_"), "tuning.json")
     t = eng.autotune(TUNE_CACHE, runs=5)
     return {"enabled": True, "bucket_capacity": t["bucket_capacity"], "bucket_target": t["bucket_target"],
-            "sort_threads": t["sort_threads"], "ms_tuned": t["ms"], "ms_default": t["ms_default"],
+            "sort_threads": t["sort_threads"], "sort_tile": t["sort_tile"], "select_threads": t["select_threads"],
+            "sort_low_bits": t["sort_low_bits"], "ms_tuned": t["ms"], "ms_default": t["ms_default"],
             "candidates": t["candidates"]}
 
 
@@ -532,6 +535,7 @@ def main():
         "launches_per_step": st["n_launches"],
         "syncs_per_step": st["n_syncs"],
         "graph_replay": bool(st["graph_replays"]),
+        "tail_sort_passes": st["tail_passes"],
         "output_pairs": out_len,
         "variants": st["n_variants"],
         "candidate_pairs": st["n_candidates"],

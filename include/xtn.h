@@ -119,6 +119,11 @@ typedef struct {
 	int64_t n_launches;    /* kernel launches of the last call */
 	int64_t n_syncs;       /* host waits on the device of the last call */
 	int64_t graph_replays; /* runs issued as one captured graph launch */
+	int64_t tail_passes;   /* 8-bit LSD passes of the last candidate sort: fewer
+	                          than the pair bits need when the select ranks
+	                          the low bits of each group in LDS, all of them
+	                          after a group outgrew the select's window on
+	                          this handle */
 } xtn_stats;
 
 /* fills *cfg with defaults: distance 1, Levenshtein, device 0 */
@@ -177,7 +182,8 @@ typedef struct {
 	double ms;                /* xtn_autotune: run time with these parameters */
 	double ms_default;        /* xtn_autotune: run time with the defaults */
 	int32_t candidates;       /* xtn_autotune: settings timed */
-	int32_t reserved;
+	int32_t sort_low_bits;    /* candidate bits below which the sort stops, ranked inside each
+	                             group by the select: 0 (default: full sort), 8, 16 or 24 */
 } xtn_tuning;
 
 int xtn_get_tuning(xtn_handle* h, xtn_tuning* out);
@@ -188,7 +194,8 @@ int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t);
  * distance and measure: each candidate setting is run `runs` times (after
  * two warm-up runs) as xtn_run_nn_resident, its output checked against the
  * default setting's; the search goes coordinate by coordinate (bucket
- * capacity and target, then the sort tile, then the select tile) and keeps
+ * capacity and target, then the sort tile, then the select tile, then the
+ * sort's low bits) and keeps
  * the fastest, the default unless another beats it by more than 1% when
  * both are timed again. With cache_path, the choice is read from that JSON
  * file when an entry for the workload (sequence count, distance, measure,

@@ -9,6 +9,12 @@ deletion variant, so one key group holds a thousand records and the
 candidates grow with its square), which also sends groups past the LDS
 bucket capacity onto the global run path.
 
+Window overflow: with the partial sort chosen (sort_low_bits 16), the sort of
+the candidate tail stops short of the low pair bits and the select ranks each
+group of equal high bits in LDS; a group too large for the select's window
+raises a flag and the run is redone with the full sort. The duplicated input
+above has such groups.
+
 Fallback: a tiny spin limit on the tail's bounded look-back waits (set from
 the host through XTN_LOOKBACK_SPIN_LIMIT) makes some wait give up and raise
 the error flag; the run is then redone with the host-checked tail, and the
@@ -55,10 +61,15 @@ def test_rounds_bounded_candidates_match_oracle():
     assert len(want) > 1_500_000  # the copies pair with each other at distance 0
     with xtn.Engine(distance=1) as eng:
         eng.set_sequences(chars=chars, offsets=offsets)
+        eng.set_tuning(sort_low_bits=16)
         n = eng.nn_resident()
         one = triples(*eng.copy_nn_output(n))
         st1 = eng.stats()
         assert np.array_equal(one, want)
+        # the copies' candidates of one row outgrow the select's window of
+        # a partly sorted tail: the run was redone fully sorted
+        idb = int(len(offsets) - 2).bit_length()
+        assert st1["tail_passes"] == (2 * idb + 7) // 8
         cap = max(st1["n_candidates"] // 5, 1 << 16)
         eng.configure(max_pairs_per_round=cap)
         n = eng.nn_resident()

@@ -1,7 +1,8 @@
 """The autotuner and the engine parameters it chooses, on the GPU: every
-bucket capacity (2048, 3072, 4096 records in LDS), bucket target and sort
-tile width gives exactly the oracle's output; xtn_autotune keeps a setting
-it has checked against the defaults and reads it back from its cache."""
+bucket capacity (2048, 3072, 4096 records in LDS), bucket target, sort tile
+width, select tile and number of low pair bits left to the select's in-group
+ranking gives exactly the oracle's output; xtn_autotune keeps a setting it
+has checked against the defaults and reads it back from its cache."""
 import json
 import os
 
@@ -20,26 +21,32 @@ def triples(p, d):
     return np.concatenate([p.astype(np.int64), d.astype(np.int64)[:, None]], axis=1)
 
 
-@pytest.mark.parametrize("cap,target,nt,tile,sel", [(2048, 1500, 256, 4096, 256), (4096, 3400, 512, 8192, 512),
-                                                    (4096, 1100, 1024, 8192, 256), (3072, 2600, 512, 4096, 512)])
-def test_settings_match_oracle(cap, target, nt, tile, sel):
+@pytest.mark.parametrize("cap,target,nt,tile,sel,low", [(2048, 1500, 256, 4096, 256, 16),
+                                                        (4096, 3400, 512, 8192, 512, 24),
+                                                        (4096, 1100, 1024, 8192, 256, 0),
+                                                        (3072, 2600, 512, 4096, 512, 8),
+                                                        (3072, 2200, 512, 4096, 256, 24)])
+def test_settings_match_oracle(cap, target, nt, tile, sel, low):
     chars, offsets = xtn.synth_nn(51, 150_000, planted=0.25, max_edits=2)
     want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    idb = int(len(offsets) - 2).bit_length()
     with xtn.Engine(distance=2) as eng:
         eng.set_sequences(chars=chars, offsets=offsets)
         eng.set_tuning(bucket_capacity=cap, bucket_target=target, sort_threads=nt, sort_tile=tile,
-                       select_threads=sel)
+                       select_threads=sel, sort_low_bits=low)
         for _ in range(3):  # eager, captured, replayed
             n = eng.nn_resident()
             assert np.array_equal(triples(*eng.copy_nn_output(n)), want)
+            assert eng.stats()["tail_passes"] == max(1, (2 * idb - low + 7) // 8)
         assert eng.tuning()["bucket_capacity"] == cap
+        assert eng.tuning()["sort_low_bits"] == low
 
 
 def test_invalid_settings_rejected():
     with xtn.Engine(distance=1) as eng:
         for kw in (dict(bucket_capacity=1000), dict(sort_threads=128), dict(bucket_capacity=2048, bucket_target=2000),
                    dict(bucket_capacity=4096, bucket_target=500), dict(sort_threads=256, sort_tile=8192),
-                   dict(sort_threads=1024, sort_tile=4096), dict(select_threads=1024)):
+                   dict(sort_threads=1024, sort_tile=4096), dict(select_threads=1024), dict(sort_low_bits=12)):
             with pytest.raises(xtn.XtnError) as e:
                 eng.set_tuning(**kw)
             assert e.value.status == 3
@@ -52,7 +59,7 @@ def test_autotune_and_cache(tmp_path):
     with xtn.Engine(distance=2) as eng:
         eng.set_sequences(chars=chars, offsets=offsets)
         t = eng.autotune(cache, runs=2)
-        assert t["from_cache"] == 0 and t["candidates"] >= 13
+        assert t["from_cache"] == 0 and t["candidates"] >= 16
         assert t["ms"] <= t["ms_default"]
         n = eng.nn_resident()
         assert np.array_equal(triples(*eng.copy_nn_output(n)), want)
@@ -63,7 +70,7 @@ def test_autotune_and_cache(tmp_path):
         eng.set_sequences(chars=chars, offsets=offsets)
         t2 = eng.autotune(cache, runs=2)
         assert t2["from_cache"] == 1
-        keys = ("bucket_capacity", "bucket_target", "sort_threads", "sort_tile", "select_threads")
+        keys = ("bucket_capacity", "bucket_target", "sort_threads", "sort_tile", "select_threads", "sort_low_bits")
         assert [t2[k] for k in keys] == [t[k] for k in keys]
         n = eng.nn_resident()
         assert np.array_equal(triples(*eng.copy_nn_output(n)), want)

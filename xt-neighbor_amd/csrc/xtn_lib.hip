@@ -103,7 +103,7 @@ struct xtn_handle {
 	struct GraphSig {
 		int64_t nseq, row_base, V, pcap, alloc_gen;
 		const void *seq_map, *set_of, *chars, *offs;
-		int32_t d, measure, bucket_cap, target, sort_nt, sort_tile, select_nt, pad;
+		int32_t d, measure, bucket_cap, target, sort_nt, sort_tile, select_nt, low;
 		bool operator==(const GraphSig& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 	};
 	hipGraph_t graph = nullptr;
@@ -117,6 +117,8 @@ struct xtn_handle {
 	int tune_sort_nt = 512;   // threads of a pair-sort tile: 256, 512 or 1024
 	int tune_sort_tile = 4096;  // keys of a pair-sort tile: 4096 or 8192
 	int tune_select_nt = 256;   // threads of a select tile (8 candidates each): 256 or 512
+	int tune_low = 0;           // pair bits left unsorted for the select's in-group ranking: 0, 8, 16, 24
+	bool tail_full = false;     // a group outgrew the select's window: full sorts until new sequences
 	double tune_ms = 0, tune_ms_default = 0;
 	bool capturing = false;
 	bool no_alloc = false;  // xtn_nn_noalloc: every buffer is a slice of the static scratch
@@ -506,7 +508,7 @@ struct CtlBlock {
 // what a run reads back: the search counters and the tail's scalars
 struct CtlHead {
 	RunCtrs rc;
-	uint32_t error, pad0;
+	uint32_t error, overflow;
 	unsigned long long n_unique, n_out;
 };
 static_assert(offsetof(CtlBlock, tc) == sizeof(RunCtrs), "the tail block follows the counters");
@@ -614,16 +616,26 @@ int bucket_fallback(xtn_handle* h, int64_t seg_start, int64_t seg_len, int rs, i
 
 struct TailPlan {
 	int64_t pcap = 0;  // candidates the launches are sized for
-	int passes = 0;    // 8-bit LSD digits over the 2 * idb pair bits
+	int passes = 0;    // 8-bit LSD digits over the pair bits from low_bits up to 2 * idb
+	int low_bits = 0;  // > 0: the select ranks the bits below in LDS (k_select_grouped)
 	int64_t sort_tiles = 0, sel_tiles = 0;
 };
+
+// LSD passes of the next candidate sort over 2 * idb pair bits: the sort
+// stops short of the low bits when asked (a key's group is its bits above
+// them); the bits left over go to the last digit's side
+int tail_passes(const xtn_handle* h, int idb) {
+	const int want = h->tail_full ? 0 : h->tune_low;
+	return std::max(1, (2 * idb - want + 7) / 8);
+}
 
 // buffers of a tail over up to pcap candidates whose output goes after the
 // out_keep pairs already held: the sort's second buffer, the outputs and the
 // look-back state
 int tail_prepare(xtn_handle* h, int64_t pcap, int idb, int64_t out_keep, TailPlan* tp) {
 	tp->pcap = pcap;
-	tp->passes = (2 * idb + 7) / 8;
+	tp->passes = tail_passes(h, idb);
+	tp->low_bits = std::max(0, 2 * idb - 8 * tp->passes);
 	const int64_t sort_tile = h->tune_sort_tile, sel_tile = (int64_t)h->tune_select_nt * SL_IT;
 	tp->sort_tiles = (pcap + sort_tile - 1) / sort_tile;
 	tp->sel_tiles = (pcap + sel_tile - 1) / sel_tile;
@@ -654,8 +666,8 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 	{
 		ProfScope ps(h, "tail_hist", 8.0 * cand);
 		const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tp.sort_tiles, 256));
-		k_tail_hist<<<g, TH_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_dev, tp.pcap, tp.passes, &cb->tc, state,
-		                                nsort + 2 * tp.sel_tiles);
+		k_tail_hist<<<g, TH_NT, 0, s>>>(h->pairs.as<uint64_t>(), n_dev, tp.pcap, tp.low_bits, tp.passes, &cb->tc,
+		                                state, nsort + 2 * tp.sel_tiles);
 		HIP_LAUNCH_CHECK(h);
 	}
 	uint64_t* a = h->pairs.as<uint64_t>();
@@ -664,15 +676,15 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 	for (int p = 0; p < tp.passes; ++p) {
 		ProfScope ps(h, "sort_pairs", 16.0 * cand);
 		uint32_t* st = state + (int64_t)p * tp.sort_tiles * RS_BINS;
-		const int nt = h->tune_sort_nt, tile = h->tune_sort_tile;
+		const int nt = h->tune_sort_nt, tile = h->tune_sort_tile, lb = tp.low_bits;
 		if (nt == 256)
-			k_onesweep<256, 16><<<sort_grid, 256, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+			k_onesweep<256, 16><<<sort_grid, 256, 0, s>>>(a, b, n_dev, tp.pcap, lb, p, &cb->tc, st, h->spin_limit);
 		else if (nt == 512 && tile == 4096)
-			k_onesweep<512, 8><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+			k_onesweep<512, 8><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, lb, p, &cb->tc, st, h->spin_limit);
 		else if (nt == 512)
-			k_onesweep<512, 16><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+			k_onesweep<512, 16><<<sort_grid, 512, 0, s>>>(a, b, n_dev, tp.pcap, lb, p, &cb->tc, st, h->spin_limit);
 		else
-			k_onesweep<1024, 8><<<sort_grid, 1024, 0, s>>>(a, b, n_dev, tp.pcap, p, &cb->tc, st, h->spin_limit);
+			k_onesweep<1024, 8><<<sort_grid, 1024, 0, s>>>(a, b, n_dev, tp.pcap, lb, p, &cb->tc, st, h->spin_limit);
 		HIP_LAUNCH_CHECK(h);
 		std::swap(a, b);
 	}
@@ -684,6 +696,7 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 	sa.distance = h->cfg.distance;
 	sa.measure = h->cfg.measure;
 	sa.row_base = row_base;
+	sa.low_bits = tp.low_bits;
 	sa.chars = h->chars.as<uint8_t>();
 	sa.offs = h->offs.as<uint32_t>();
 	sa.seq_map = seq_map;
@@ -700,7 +713,8 @@ int issue_tail(xtn_handle* h, const TailPlan& tp, int idb, const uint32_t* seq_m
 		// kept triples written
 		ProfScope ps(h, "select", 8.0 * cand + 40.0 * (double)h->last_unique + 9.0 * (double)h->last_out);
 		const unsigned g = (unsigned)std::max<int64_t>(1, tp.sel_tiles);
-		select_kernel(h->cfg.distance, h->cfg.measure, h->tune_select_nt)<<<g, h->tune_select_nt, 0, s>>>(sa);
+		select_kernel(h->cfg.distance, h->cfg.measure, h->tune_select_nt, tp.low_bits > 0)<<<g, h->tune_select_nt, 0,
+		                                                                                   s>>>(sa);
 		HIP_LAUNCH_CHECK(h);
 	}
 	return XTN_OK;
@@ -1213,6 +1227,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				sig.sort_nt = h->tune_sort_nt;
 				sig.sort_tile = h->tune_sort_tile;
 				sig.select_nt = h->tune_select_nt;
+				sig.low = h->tail_full ? 0 : h->tune_low;
 			}
 			bool via_graph = false;
 			if (graph_ok && h->graph_exec && sig == h->graph_sig) {
@@ -1259,7 +1274,10 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				TRY(issue_all());
 			}
 			TRY(read_head(h, &head));
-			const bool clean = !head.error && (int64_t)head.rc.n_cand <= pcap && head.rc.append == 0;
+			const bool clean =
+			    !head.error && !head.overflow && (int64_t)head.rc.n_cand <= pcap && head.rc.append == 0;
+			if (head.overflow)
+				h->tail_full = true;  // a group outgrew the select's window: full sorts from now on
 			if (via_graph) {
 				st.graph_replays += 1;
 				if (!clean) {
@@ -1277,6 +1295,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				h->tail_safe = true;
 				continue;
 			}
+			if (head.overflow)
+				continue;  // the round is redone with the full sort
 			if (clean) {
 				n_pairs = (int64_t)head.rc.n_cand;
 				done = true;
@@ -1329,6 +1349,10 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				h->tail_safe = true;  // see above; the round is redone
 				continue;
 			}
+			if (head.overflow) {
+				h->tail_full = true;  // see above; the round is redone
+				continue;
+			}
 		}
 		st.n_candidates += n_pairs;
 		st.n_unique += (int64_t)head.n_unique;
@@ -1336,6 +1360,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		max_round = std::max(max_round, n_pairs);
 		++st.n_rounds;
 		st.tail_checked = h->tail_safe ? 1 : 0;
+		st.tail_passes = h->tail_safe ? (2 * idb + 7) / 8 : tail_passes(h, idb);
 		rounds.pop_front();
 	}
 	h->last_pairs = max_round;
@@ -1749,6 +1774,7 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 	h->prefix_len.swap(plen);
 	h->vprefix.clear();
 	h->vprefix_d = 0;
+	h->tail_full = false;
 	return XTN_OK;
 }
 
@@ -1905,6 +1931,7 @@ int xtn_perform_sets(xtn_handle* h, const uint32_t* set_offsets, int32_t n_sets,
 		tot.n_syncs += st.n_syncs;
 		tot.graph_replays += st.graph_replays;
 		tot.tail_checked |= st.tail_checked;
+		tot.tail_passes = std::max(tot.tail_passes, st.tail_passes);
 	};
 	std::vector<xtn_int2> hp;
 	std::vector<int8_t> hd;
@@ -1972,6 +1999,7 @@ int xtn_get_tuning(xtn_handle* h, xtn_tuning* out) {
 	out->sort_threads = h->tune_sort_nt;
 	out->sort_tile = h->tune_sort_tile;
 	out->select_threads = h->tune_select_nt;
+	out->sort_low_bits = h->tune_low;
 	out->ms = h->tune_ms;
 	out->ms_default = h->tune_ms_default;
 	return XTN_OK;
@@ -1983,17 +2011,20 @@ int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t) {
 	const int c = t->bucket_capacity, nt = t->sort_threads, tile = t->sort_tile;
 	const bool sort_ok = (nt == 256 && tile == 4096) || (nt == 512 && (tile == 4096 || tile == 8192)) ||
 	                     (nt == 1024 && tile == 8192);
+	const int lb = t->sort_low_bits;
 	if ((c != 2048 && c != 3072 && c != 4096) || !sort_ok ||
 	    (t->select_threads != 256 && t->select_threads != 512) || t->bucket_target < c / 4 ||
-	    t->bucket_target > c * 9 / 10)
+	    t->bucket_target > c * 9 / 10 || (lb != 0 && lb != 8 && lb != 16 && lb != 24))
 		return fail(h, XTN_ERR_ARG,
-		            "invalid tuning (capacity %d, target %d, sort threads %d, sort tile %d, select threads %d)", c,
-		            t->bucket_target, nt, tile, t->select_threads);
+		            "invalid tuning (capacity %d, target %d, sort threads %d, sort tile %d, select threads %d, "
+		            "low bits %d)",
+		            c, t->bucket_target, nt, tile, t->select_threads, lb);
 	h->tune_cap = c;
 	h->tune_target = t->bucket_target;
 	h->tune_sort_nt = nt;
 	h->tune_sort_tile = tile;
 	h->tune_select_nt = t->select_threads;
+	h->tune_low = lb;
 	return XTN_OK;
 }
 
@@ -2075,15 +2106,16 @@ static bool tune_cache_read(const char* path, const std::string& sig, xtn_tuning
 			const char* p = strstr(line, name);
 			return p && sscanf(p + strlen(name), " : %lf", v) == 1;
 		};
-		double c, tg, nt, tl, sn, ms, msd;
+		double c, tg, nt, tl, sn, lb, ms, msd;
 		if (num("\"bucket_capacity\"", &c) && num("\"bucket_target\"", &tg) && num("\"sort_threads\"", &nt) &&
-		    num("\"sort_tile\"", &tl) && num("\"select_threads\"", &sn) && num("\"ms\"", &ms) &&
-		    num("\"ms_default\"", &msd)) {
+		    num("\"sort_tile\"", &tl) && num("\"select_threads\"", &sn) && num("\"sort_low_bits\"", &lb) &&
+		    num("\"ms\"", &ms) && num("\"ms_default\"", &msd)) {
 			t->bucket_capacity = (int32_t)c;
 			t->bucket_target = (int32_t)tg;
 			t->sort_threads = (int32_t)nt;
 			t->sort_tile = (int32_t)tl;
 			t->select_threads = (int32_t)sn;
+			t->sort_low_bits = (int32_t)lb;
 			t->ms = ms;
 			t->ms_default = msd;
 			found = true;
@@ -2110,9 +2142,10 @@ static bool tune_cache_write(const char* path, const std::string& sig, const xtn
 	char buf[512];
 	snprintf(buf, sizeof(buf),
 	         "  {\"workload\": \"%s\", \"bucket_capacity\": %d, \"bucket_target\": %d, \"sort_threads\": %d, "
-	         "\"sort_tile\": %d, \"select_threads\": %d, \"ms\": %.4f, \"ms_default\": %.4f}",
-	         sig.c_str(), t.bucket_capacity, t.bucket_target, t.sort_threads, t.sort_tile, t.select_threads, t.ms,
-	         t.ms_default);
+	         "\"sort_tile\": %d, \"select_threads\": %d, \"sort_low_bits\": %d, \"ms\": %.4f, "
+	         "\"ms_default\": %.4f}",
+	         sig.c_str(), t.bucket_capacity, t.bucket_target, t.sort_threads, t.sort_tile, t.select_threads,
+	         t.sort_low_bits, t.ms, t.ms_default);
 	keep.push_back(buf);
 	const std::string tmp = std::string(path) + ".tmp";
 	FILE* f = fopen(tmp.c_str(), "w");
@@ -2175,10 +2208,10 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 	// its output must be the default setting's
 	int32_t runs_now = runs;
 	struct Setting {
-		int cap, target, sort_nt, sort_tile, select_nt;
+		int cap, target, sort_nt, sort_tile, select_nt, low;
 		bool operator==(const Setting& o) const {
 			return cap == o.cap && target == o.target && sort_nt == o.sort_nt && sort_tile == o.sort_tile &&
-			       select_nt == o.select_nt;
+			       select_nt == o.select_nt && low == o.low;
 		}
 	};
 	uint64_t dig_def = 0;
@@ -2190,6 +2223,7 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 		h->tune_sort_nt = st.sort_nt;
 		h->tune_sort_tile = st.sort_tile;
 		h->tune_select_nt = st.select_nt;
+		h->tune_low = st.low;
 		std::vector<double> t;
 		for (int r = 0; r < runs_now + 2; ++r) {
 			memset(&h->stats, 0, sizeof(h->stats));
@@ -2206,33 +2240,40 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 			have_def = true;
 		} else if (dig != dig_def) {
 			return fail(h, XTN_ERR_HIP,
-			            "autotune: capacity %d, target %d, sort %dx%d, select %d changed the output", st.cap,
-			            st.target, st.sort_nt, st.sort_tile, st.select_nt);
+			            "autotune: capacity %d, target %d, sort %dx%d, select %d, low bits %d changed the output",
+			            st.cap, st.target, st.sort_nt, st.sort_tile, st.select_nt, st.low);
 		}
 		++tried;
 		return XTN_OK;
 	};
-	const Setting def{3072, 2200, 512, 4096, 256};
+	const Setting def{3072, 2200, 512, 4096, 256, 0};
 	double ms_def = 0;
 	TRY(time_setting(def, &ms_def));
 	Setting cur = def;
 	double cur_ms = ms_def;
 	// coordinate descent: bucket capacity and target, then the sort tile,
-	// then the select tile, each step keeping the best so far
+	// then the select tile, then the sort's low bits, each step keeping the
+	// best so far
 	const int caps[3] = {3072, 4096, 2048};
 	const int targets[3][3] = {{1840, 2200, 2600}, {2450, 2950, 3480}, {1230, 1470, 1740}};
-	std::vector<Setting> step1, step2, step3;
+	std::vector<Setting> step1;
 	for (int ci = 0; ci < 3; ++ci)
 		for (int tg : targets[ci])
-			step1.push_back({caps[ci], tg, def.sort_nt, def.sort_tile, def.select_nt});
-	for (const std::vector<Setting>* group : {&step1, &step2, &step3}) {
-		std::vector<Setting> cands = *group;
-		if (group == &step2)
-			cands = {{cur.cap, cur.target, 256, 4096, cur.select_nt}, {cur.cap, cur.target, 512, 4096, cur.select_nt},
-			         {cur.cap, cur.target, 512, 8192, cur.select_nt}, {cur.cap, cur.target, 1024, 8192, cur.select_nt}};
-		if (group == &step3)
-			cands = {{cur.cap, cur.target, cur.sort_nt, cur.sort_tile, 256},
-			         {cur.cap, cur.target, cur.sort_nt, cur.sort_tile, 512}};
+			step1.push_back({caps[ci], tg, def.sort_nt, def.sort_tile, def.select_nt, def.low});
+	for (int step = 1; step <= 4; ++step) {
+		std::vector<Setting> cands = step1;
+		const Setting c = cur;
+		if (step == 2)
+			cands = {{c.cap, c.target, 256, 4096, c.select_nt, c.low}, {c.cap, c.target, 512, 4096, c.select_nt, c.low},
+			         {c.cap, c.target, 512, 8192, c.select_nt, c.low}, {c.cap, c.target, 1024, 8192, c.select_nt, c.low}};
+		if (step == 3)
+			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, 256, c.low},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, 512, c.low}};
+		if (step == 4)
+			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 0},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 8},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 16},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 24}};
 		for (const Setting& st : cands) {
 			if (st == cur)
 				continue;
@@ -2267,6 +2308,7 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 	best.sort_threads = cur.sort_nt;
 	best.sort_tile = cur.sort_tile;
 	best.select_threads = cur.select_nt;
+	best.sort_low_bits = cur.low;
 	best.ms = cur_ms;
 	best.ms_default = ms_def;
 	best.candidates = tried;
@@ -2548,7 +2590,9 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 	pcap = std::min<int64_t>(pcap, (int64_t)LB_VAL);
 	CtlHead head{};
 	TailPlan tp;
-	for (int attempt = 0; attempt < 2; ++attempt) {
+	bool grown = false;
+	w->tail_full = false;
+	for (;;) {
 		TRY(ensure(w, w->pairs, sizeof(uint64_t) * (size_t)pcap));
 		TRY(tail_prepare(w, pcap, idb, 0, &tp));
 		TRY(zero_ctl(w, true, true));
@@ -2561,14 +2605,22 @@ int serve_query(xtn_server* sv, xtn_handle* w, const ServeRequest& rq, ServeResu
 		TRY(read_head(w, &head));
 		if (head.error)
 			return fail(w, XTN_ERR_HIP, "a look-back wait of the query's candidate tail exceeded its limit");
-		if ((int64_t)head.rc.n_cand <= pcap)
+		if ((int64_t)head.rc.n_cand > pcap) {
+			if (grown || head.rc.n_cand > (unsigned long long)LB_VAL)
+				return fail(w, XTN_ERR_CAPACITY, "query candidates exceed the per-query limit (%llu)",
+				            (unsigned long long)head.rc.n_cand);
+			pcap = (int64_t)head.rc.n_cand + (int64_t)head.rc.n_cand / 4;
+			pcap = std::min<int64_t>(pcap, (int64_t)LB_VAL);
+			grown = true;
+			continue;
+		}
+		if (!head.overflow)
 			break;
-		if (attempt == 1 || head.rc.n_cand > (unsigned long long)LB_VAL)
-			return fail(w, XTN_ERR_CAPACITY, "query candidates exceed the per-query limit (%llu)",
-			            (unsigned long long)head.rc.n_cand);
-		pcap = (int64_t)head.rc.n_cand + (int64_t)head.rc.n_cand / 4;
-		pcap = std::min<int64_t>(pcap, (int64_t)LB_VAL);
+		// a group outgrew the select's window: this query's tail again, fully
+		// sorted (the full sort never sets the flag)
+		w->tail_full = true;
 	}
+	w->tail_full = false;
 	const double r = (double)head.rc.n_cand / (double)std::max<int64_t>(V, 1);
 	w->serve_ratio = w->serve_ratio > 0 ? std::max(r, 0.75 * w->serve_ratio + 0.25 * r) : std::max(r, 0.01);
 	const int64_t nout = (int64_t)head.n_out;

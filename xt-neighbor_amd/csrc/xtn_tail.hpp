@@ -52,7 +52,7 @@ constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;  // s_sleep(1) rounds a wait may ta
 // first so that one small copy reads them back
 struct TailCtl {
 	uint32_t error;                            // a look-back spin hit its limit
-	uint32_t pad0;
+	uint32_t overflow;                         // a select tile's group range outgrew its LDS
 	unsigned long long n_unique;               // unique candidates verified
 	unsigned long long n_out;                  // pairs within the distance
 	uint32_t pad1[12];
@@ -174,11 +174,12 @@ constexpr int TH_NT = 512;
 constexpr int TH_IT = 8;
 
 // Global digit counts of the first min(*n_dev, cap) keys for `passes` 8-bit
-// digits (digit p = bits [8p, 8p + 8)), grid-stride over tiles; also zeroes
+// digits (digit p = bits [lbits + 8p, lbits + 8p + 8)), grid-stride over
+// tiles; also zeroes
 // zero[0, nzero) (the look-back state of the passes and the select).
 __global__ __launch_bounds__(TH_NT) void k_tail_hist(const uint64_t* __restrict__ keys,
                                                      const unsigned long long* __restrict__ n_dev,
-                                                     int64_t cap, int passes, TailCtl* __restrict__ ctl,
+                                                     int64_t cap, int lbits, int passes, TailCtl* __restrict__ ctl,
                                                      uint32_t* __restrict__ zero, int64_t nzero) {
 	__shared__ uint32_t s_h[TAIL_MAX_PASSES][RS_BINS];
 	const int t = threadIdx.x;
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(TH_NT) void k_tail_hist(const uint64_t* __restrict_
 			const int64_t q = base + (int64_t)r * TH_NT + t;
 			if (q < n)
 				for (int p = 0; p < passes; ++p)
-					atomicAdd(&s_h[p][(uint32_t)(k[r] >> (8 * p)) & 0xFFu], 1u);
+					atomicAdd(&s_h[p][(uint32_t)(k[r] >> (lbits + 8 * p)) & 0xFFu], 1u);
 		}
 	}
 	__syncthreads();
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(TH_NT) void k_tail_hist(const uint64_t* __restrict_
 	}
 }
 
-// One stable LSD pass over the digit at `shift` (pass index `pass`):
+// One stable LSD pass over the digit at bits lbits + 8 * pass:
 // keys_in -> keys_out. state: [tiles][RS_BINS] status words of this pass,
 // zeroed. The grid covers the capacity; workgroups past the actual count
 // exit at once.
@@ -224,7 +225,7 @@ template <int NT, int IT>
 __global__ __launch_bounds__(NT) void k_onesweep(const uint64_t* __restrict__ keys_in,
                                                  uint64_t* __restrict__ keys_out,
                                                  const unsigned long long* __restrict__ n_dev, int64_t cap,
-                                                 int pass, TailCtl* __restrict__ ctl,
+                                                 int lbits, int pass, TailCtl* __restrict__ ctl,
                                                  uint32_t* __restrict__ state, uint32_t spin_limit) {
 	constexpr int TILE = NT * IT;
 	constexpr int SEG = IT * WAVE;
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(NT) void k_onesweep(const uint64_t* __restrict__ ke
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
-	const int shift = 8 * pass;
+	const int shift = lbits + 8 * pass;
 	const int64_t n = tail_count(n_dev, cap);
 	const int64_t tile = blockIdx.x;
 	if (tile * TILE >= n)  // uniform
@@ -301,6 +302,7 @@ struct SelectArgs {
 	int32_t distance;
 	int32_t measure;
 	uint32_t row_base;
+	int32_t low_bits;  // the sort left the bits below this unsorted (k_select_grouped); 0: fully sorted
 	const uint8_t* chars;
 	const uint32_t* offs;
 	const uint32_t* seq_map;  // nullable: id -> row
@@ -314,56 +316,66 @@ struct SelectArgs {
 	uint32_t spin_limit;
 };
 
-// Workgroup b handles tile b of SL_TILE sorted candidates: the first of each
-// run of equal pairs (the key before the tile decides for its first key) is
-// compacted in LDS in order, verified one pair per lane, and the pairs within
-// the distance go out at the tile's offset (look-back over the kept counts of
-// the tiles before it) plus their rank in the tile, so the output stays
-// sorted. The last tile writes the unique and kept totals.
-// D: the distance's band fixed at compile time (pair_distance_d). NT
-// threads, a tile of NT * SL_IT candidates.
-template <int D, int NT>
-__global__ __launch_bounds__(NT) void k_select(SelectArgs a) {
-	constexpr int SL_TILE = NT * SL_IT;
-	__shared__ uint64_t s_k[SL_TILE];  // the tile as loaded
-	__shared__ uint64_t s_u[SL_TILE];  // its first-of-run pairs, in order
-	__shared__ int8_t s_d[SL_TILE];
-	__shared__ uint32_t s_tmp[NT / WAVE + 1];
-	__shared__ uint32_t s_excl;
-	const int t = threadIdx.x;
-	const int64_t n = tail_count(a.n_dev, a.cap);
-	const int64_t ntiles = (n + SL_TILE - 1) / SL_TILE;
-	const uint64_t idmask = (1ull << a.idb) - 1ull;
-	const int64_t tile = blockIdx.x;
-	if (tile >= ntiles)  // uniform
-		return;
-	const int64_t base = tile * SL_TILE;
+// Ordered ranks over R * NT positions laid out in rows, position
+// q = r * NT + threadIdx.x (row r: NT consecutive positions, one per
+// thread, so a wave touches consecutive LDS words). flags bit r: this
+// thread's position in row r is marked. ex[r] = marked positions before it,
+// in position order; returns the total. A ballot per row and wave gives the
+// count below each lane, one wave scans the per-(row, wave) counts in order.
+// s_cc: R * NT / WAVE + 1 words. Every thread calls it; it ends in a barrier.
+template <int NT, int R>
+__device__ __forceinline__ uint32_t striped_rank(uint32_t flags, uint32_t (&ex)[R], uint32_t* s_cc) {
+	constexpr int NW = NT / WAVE;
+	constexpr int E = R * NW;
+	constexpr int C = (E + WAVE - 1) / WAVE;
+	const int lane = lane_id();
+	const int w = threadIdx.x / WAVE;
+	const uint64_t below = (1ull << lane) - 1ull;
+	uint32_t within[R];
 #pragma unroll
-	for (int r = 0; r < SL_IT; ++r) {
-		const int64_t q = base + (int64_t)r * NT + t;
-		s_k[r * NT + t] = q < n ? a.keys[q] : ~0ull;
+	for (int r = 0; r < R; ++r) {
+		const uint64_t b = __ballot((flags >> r) & 1u);
+		within[r] = (uint32_t)__popcll(b & below);
+		if (lane == 0)
+			s_cc[r * NW + w] = (uint32_t)__popcll(b);
 	}
-	const uint64_t before = base > 0 ? a.keys[base - 1] : ~0ull;
 	__syncthreads();
-	// thread t owns tile positions [t * SL_IT, (t + 1) * SL_IT)
-	uint32_t uflags = 0, cu = 0;
+	if (w == 0) {
+		uint32_t v[C];
+		uint32_t sum = 0;
 #pragma unroll
-	for (int i = 0; i < SL_IT; ++i) {
-		const int q = t * SL_IT + i;
-		const uint64_t prev = q == 0 ? before : s_k[q - 1];
-		if (base + q < n && s_k[q] != prev) {
-			uflags |= 1u << i;
-			++cu;
+		for (int c = 0; c < C; ++c) {
+			const int idx = lane * C + c;
+			v[c] = idx < E ? s_cc[idx] : 0u;
+			sum += v[c];
 		}
-	}
-	uint32_t nu;
-	uint32_t pos = block_excl_scan<NT>(cu, s_tmp, &nu);
+		const uint32_t inc = wave_incl_scan(sum);
+		uint32_t e = inc - sum;
 #pragma unroll
-	for (int i = 0; i < SL_IT; ++i)
-		if (uflags & (1u << i))
-			s_u[pos++] = s_k[t * SL_IT + i];
+		for (int c = 0; c < C; ++c) {
+			const int idx = lane * C + c;
+			if (idx < E)
+				s_cc[idx] = e;
+			e += v[c];
+		}
+		if (lane == WAVE - 1)
+			s_cc[E] = inc;
+	}
 	__syncthreads();
-	for (uint32_t q = t; q < nu; q += NT) {
+#pragma unroll
+	for (int r = 0; r < R; ++r)
+		ex[r] = s_cc[r * NW + w] + within[r];
+	const uint32_t total = s_cc[E];
+	__syncthreads();
+	return total;
+}
+
+// the unique pairs s_u[0, nu) of a tile, verified one pair per lane: s_d[q]
+// = the distance, or -1 past it (or across query sets)
+template <int D, int NT>
+__device__ __forceinline__ void select_verify(const SelectArgs& a, const uint64_t* s_u, int8_t* s_d, uint32_t nu) {
+	const uint64_t idmask = (1ull << a.idb) - 1ull;
+	for (uint32_t q = threadIdx.x; q < nu; q += NT) {
 		const uint64_t p = s_u[q];
 		const uint32_t ia = (uint32_t)(p >> a.idb), ib = (uint32_t)(p & idmask);
 		int8_t dv = -1;
@@ -381,17 +393,25 @@ __global__ __launch_bounds__(NT) void k_select(SelectArgs a) {
 		s_d[q] = dv;
 	}
 	__syncthreads();
-	uint32_t kflags = 0, ck = 0;
+}
+
+// The tile's unique and kept counts published and the tiles before it looked
+// back on, then the kept pairs of s_u[0, nu) written in order at the tile's
+// offset (rows of NT positions, so consecutive lanes write consecutive
+// slots). The last tile writes the totals.
+template <int NT, int R>
+__device__ __forceinline__ void select_emit(const SelectArgs& a, int64_t tile, int64_t ntiles, const uint64_t* s_u,
+                                            const int8_t* s_d, uint32_t nu, uint32_t* s_cc, uint32_t* s_excl) {
+	const int t = threadIdx.x;
+	uint32_t kflags = 0;
 #pragma unroll
-	for (int i = 0; i < SL_IT; ++i) {
-		const uint32_t q = (uint32_t)(t * SL_IT + i);
-		if (q < nu && s_d[q] >= 0) {
-			kflags |= 1u << i;
-			++ck;
-		}
+	for (int r = 0; r < R; ++r) {
+		const uint32_t q = (uint32_t)(r * NT + t);
+		if (q < nu && s_d[q] >= 0)
+			kflags |= 1u << r;
 	}
-	uint32_t nk;
-	const uint32_t kpos = block_excl_scan<NT>(ck, s_tmp, &nk);
+	uint32_t kpos[R];
+	const uint32_t nk = striped_rank<NT, R>(kflags, kpos, s_cc);
 	if (t < WAVE) {  // the first wave publishes and looks back
 		uint64_t* st = a.state + tile;
 		const uint64_t mine = ((uint64_t)nu << LB2_SHIFT) | nk;
@@ -407,7 +427,7 @@ __global__ __launch_bounds__(NT) void k_select(SelectArgs a) {
 				lb_store64(st, LB2_INC | (excl + mine));
 		}
 		if (t == 0) {
-			s_excl = (uint32_t)(excl & ((1ull << LB2_SHIFT) - 1ull));
+			*s_excl = (uint32_t)(excl & ((1ull << LB2_SHIFT) - 1ull));
 			if (tile == ntiles - 1) {
 				const uint64_t tot = excl + mine;
 				a.ctl->n_unique = tot >> LB2_SHIFT;
@@ -416,39 +436,324 @@ __global__ __launch_bounds__(NT) void k_select(SelectArgs a) {
 		}
 	}
 	__syncthreads();
-	uint64_t o = (uint64_t)s_excl + kpos;
+	const uint64_t idmask = (1ull << a.idb) - 1ull;
+	const uint64_t o = *s_excl;
 #pragma unroll
-	for (int i = 0; i < SL_IT; ++i)
-		if (kflags & (1u << i)) {
-			const uint64_t p = s_u[t * SL_IT + i];
-			a.out_pairs[o] = make_int2((int)(p >> a.idb), (int)(p & idmask));
-			a.out_dist[o] = s_d[t * SL_IT + i];
-			++o;
+	for (int r = 0; r < R; ++r)
+		if (kflags & (1u << r)) {
+			const uint32_t q = (uint32_t)(r * NT + t);
+			const uint64_t p = s_u[q];
+			a.out_pairs[o + kpos[r]] = make_int2((int)(p >> a.idb), (int)(p & idmask));
+			a.out_dist[o + kpos[r]] = s_d[q];
 		}
 }
 
-// the select kernel of a run: the band of its distance (Hamming needs none),
-// NT threads per tile
-typedef void (*SelectFn)(SelectArgs);
+// Workgroup b handles tile b of SL_TILE fully sorted candidates: the first of
+// each run of equal pairs (the key before the tile decides for its first key)
+// is compacted in LDS in order, verified one pair per lane, and the pairs
+// within the distance go out at the tile's offset (look-back over the kept
+// counts of the tiles before it) plus their rank in the tile, so the output
+// stays sorted. D: the distance's band fixed at compile time
+// (pair_distance_d). NT threads, a tile of NT * SL_IT candidates.
+template <int D, int NT>
+__global__ __launch_bounds__(NT) void k_select(SelectArgs a) {
+	constexpr int SL_TILE = NT * SL_IT;
+	__shared__ uint64_t s_k[SL_TILE];  // the tile as loaded
+	__shared__ uint64_t s_u[SL_TILE];  // its first-of-run pairs, in order
+	__shared__ int8_t s_d[SL_TILE];
+	__shared__ uint32_t s_cc[SL_IT * NT / WAVE + 1];
+	__shared__ uint32_t s_excl;
+	const int t = threadIdx.x;
+	const int64_t n = tail_count(a.n_dev, a.cap);
+	const int64_t ntiles = (n + SL_TILE - 1) / SL_TILE;
+	const int64_t tile = blockIdx.x;
+	if (tile >= ntiles)  // uniform
+		return;
+	const int64_t base = tile * SL_TILE;
+	uint64_t k[SL_IT];
+#pragma unroll
+	for (int r = 0; r < SL_IT; ++r) {
+		const int64_t q = base + (int64_t)r * NT + t;
+		k[r] = q < n ? a.keys[q] : ~0ull;
+		s_k[r * NT + t] = k[r];
+	}
+	const uint64_t before = base > 0 ? a.keys[base - 1] : ~0ull;
+	__syncthreads();
+	uint32_t uflags = 0;
+#pragma unroll
+	for (int r = 0; r < SL_IT; ++r) {
+		const int q = r * NT + t;
+		const uint64_t prev = q == 0 ? before : s_k[q - 1];
+		if (base + q < n && k[r] != prev)
+			uflags |= 1u << r;
+	}
+	uint32_t pos[SL_IT];
+	const uint32_t nu = striped_rank<NT, SL_IT>(uflags, pos, s_cc);
+#pragma unroll
+	for (int r = 0; r < SL_IT; ++r)
+		if (uflags & (1u << r))
+			s_u[pos[r]] = k[r];
+	__syncthreads();
+	select_verify<D, NT>(a, s_u, s_d, nu);
+	select_emit<NT, SL_IT>(a, tile, ntiles, s_u, s_d, nu, s_cc, &s_excl);
+}
+
+// Bitonic sort of s[0, p2) in LDS (p2 a power of two), ascending
 template <int NT>
+__device__ __forceinline__ void lds_bitonic(uint64_t* s, int p2) {
+	for (int k = 2; k <= p2; k <<= 1)
+		for (int j = k >> 1; j > 0; j >>= 1) {
+			for (int i = threadIdx.x; i < (p2 >> 1); i += NT) {
+				const int x = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+				const int y = x + j;
+				const uint64_t u = s[x], v = s[y];
+				if ((u > v) == ((x & k) == 0)) {
+					s[x] = v;
+					s[y] = u;
+				}
+			}
+			__syncthreads();
+		}
+}
+
+constexpr int SG_SCAN = 96;  // steps a key's look for an earlier equal key may take
+
+// The same over candidates sorted only by their bits above the low a.low_bits
+// (a group: the candidates of equal key >> low_bits, in no order inside).
+// Tile b owns the groups that start in [b * T, (b + 1) * T) (T = NT * SL_IT)
+// and reads on past its end to the next group start, up to a window of 2T;
+// a group range that does not fit raises the overflow flag (the host then
+// redoes the tail fully sorted) and the tile counts nothing. Positions are
+// laid out in rows of NT (position r * NT + thread).
+//   1. the groups of the range are numbered by an ordered rank of their starts
+//   2. a key is the first of its value when no earlier key of its group
+//      equals it; the look goes backwards from the key (duplicates of a pair
+//      interleave with the few other pairs of the group, so an equal key is
+//      usually a few steps away) and gives up after SG_SCAN steps, counting
+//      the key as a first and the tile as hard
+//   3. the firsts are compacted in position order, so each group's firsts
+//      are contiguous; in an easy tile a first's place is its group's start
+//      plus the smaller firsts of the group (at most SG_SCAN of them)
+//   4. a hard tile sorts its compacted firsts whole in LDS (bitonic) and
+//      keeps the first of each run
+// Either way the unique pairs then sit in LDS in ascending order, as after a
+// full sort.
+template <int D, int NT>
+__global__ __launch_bounds__(NT) void k_select_grouped(SelectArgs a) {
+	constexpr int T = NT * SL_IT;
+	constexpr int WIN = 2 * T;
+	constexpr int R = WIN / NT;  // rows of the window
+	constexpr uint32_t NONE = 0xFFFFFFFFu;
+	__shared__ uint64_t s_k[WIN];       // the window as loaded, then its firsts, then its unique pairs in order
+	__shared__ uint16_t s_gs[WIN + 1];  // group g starts at s_gs[g] (s_gs[groups] = end), then its firsts do
+	__shared__ int8_t s_d[WIN];
+	__shared__ uint32_t s_cc[R * NT / WAVE + 1];
+	__shared__ uint32_t s_excl, s_start, s_end, s_hard;
+	static_assert(WIN <= 65535, "positions in 16 bits");
+	const int t = threadIdx.x;
+	const int L = a.low_bits;
+	const int64_t n = tail_count(a.n_dev, a.cap);
+	const int64_t ntiles = (n + T - 1) / T;
+	const int64_t tile = blockIdx.x;
+	if (tile >= ntiles)  // uniform
+		return;
+	const int64_t base = tile * T;
+	const int tile_n = (int)(n - base < T ? n - base : T);
+	if (t == 0) {
+		s_start = NONE;
+		s_end = base + T >= n ? (uint32_t)tile_n : NONE;
+		s_hard = 0;
+	}
+	uint64_t k[R];
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		const int q = r * NT + t;
+		k[r] = r < SL_IT && q < tile_n ? a.keys[base + q] : ~0ull;
+		if (r < SL_IT)
+			s_k[q] = k[r];
+	}
+	__syncthreads();
+	// the range's end: the first group start at or after (b + 1) T, read on a
+	// row at a time
+	if (base + T < n) {
+#pragma unroll
+		for (int r = SL_IT; r < R; ++r) {
+			const int e = r * NT + t;
+			const int64_t qg = base + e;
+			bool stop = qg >= n;
+			if (!stop) {
+				k[r] = a.keys[qg];
+				stop = (k[r] >> L) != (a.keys[qg - 1] >> L);
+			}
+			s_k[e] = k[r];
+			if (stop)
+				atomicMin(&s_end, qg < n ? (uint32_t)e : (uint32_t)(n - base));
+			if (__syncthreads_or(stop))
+				break;
+		}
+	}
+	// the range's start: the first group start in [b T, (b + 1) T)
+	const uint64_t before = base > 0 ? a.keys[base - 1] : 0ull;
+#pragma unroll
+	for (int r = 0; r < SL_IT; ++r) {
+		const int q = r * NT + t;
+		if (q < tile_n) {
+			const bool bnd = q == 0 ? (base == 0 || (k[0] >> L) != (before >> L)) : (k[r] >> L) != (s_k[q - 1] >> L);
+			if (bnd) {
+				atomicMin(&s_start, (uint32_t)q);
+				break;
+			}
+		}
+	}
+	__syncthreads();
+	int lo = (int)s_start, hi = (int)s_end;
+	if (s_start == NONE) {
+		lo = hi = 0;  // every key here belongs to a group of an earlier tile
+	} else if (s_end == NONE) {
+		if (t == 0)
+			atomicOr(&a.ctl->overflow, 1u);
+		lo = hi = 0;
+	}
+	// 1. group starts, numbered in order; gid[r] = the group of position r
+	uint32_t in = 0, bflags = 0;
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		const int q = r * NT + t;
+		if (q >= lo && q < hi) {
+			in |= 1u << r;
+			if (q == lo || (k[r] >> L) != (s_k[q - 1] >> L))
+				bflags |= 1u << r;
+		}
+	}
+	uint32_t gid[R];
+	const uint32_t ngroups = striped_rank<NT, R>(bflags, gid, s_cc);
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		if (bflags & (1u << r))
+			s_gs[gid[r]] = (uint16_t)(r * NT + t);
+		else
+			gid[r] -= 1u;  // a position after its group's start
+	}
+	if (t == 0)
+		s_gs[ngroups] = (uint16_t)hi;
+	__syncthreads();
+	// 2. firsts of their value
+	uint32_t fl = 0;
+	bool hard = false;
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		if (in & (1u << r)) {
+			const int q = r * NT + t;
+			const int gs = s_gs[gid[r]];
+			const int stop = q - SG_SCAN > gs ? q - SG_SCAN : gs;
+			int p = q - 1;
+			while (p >= stop && s_k[p] != k[r])
+				--p;
+			if (p < stop) {  // no equal key seen
+				hard = hard || stop > gs;
+				fl |= 1u << r;
+			}
+		}
+	}
+	if (hard)
+		s_hard = 1;
+	// 3. the firsts compacted in position order; group g's firsts start at
+	// s_gs[g] from here on
+	uint32_t c[R];
+	const uint32_t nc = striped_rank<NT, R>(fl, c, s_cc);
+#pragma unroll
+	for (int r = 0; r < R; ++r) {
+		if (bflags & (1u << r))
+			s_gs[gid[r]] = (uint16_t)c[r];  // firsts before the group's start
+		if (fl & (1u << r))
+			s_k[c[r]] = k[r];
+	}
+	if (t == 0)
+		s_gs[ngroups] = (uint16_t)nc;
+	__syncthreads();
+	{
+		bool big = false;  // a group of many firsts makes the tile hard as well
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			if (bflags & (1u << r))
+				big = big || (int)s_gs[gid[r] + 1] - (int)s_gs[gid[r]] > SG_SCAN;
+		if (big)
+			s_hard = 1;
+	}
+	__syncthreads();
+	uint32_t nu = nc;
+	if (!s_hard) {  // uniform
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			if (fl & (1u << r)) {
+				const int cs = s_gs[gid[r]], ce = s_gs[gid[r] + 1];
+				uint32_t less = 0;
+				for (int p = cs; p < ce; ++p)
+					less += s_k[p] < k[r] ? 1u : 0u;
+				c[r] = (uint32_t)cs + less;
+			}
+		}
+		__syncthreads();
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			if (fl & (1u << r))
+				s_k[c[r]] = k[r];
+		__syncthreads();
+	} else {
+		// 4. the firsts padded to a power of two with keys above every pair,
+		// sorted whole, the first of each run kept
+		int p2 = 2;
+		while (p2 < (int)nc)
+			p2 <<= 1;
+		for (int q = (int)nc + t; q < p2; q += NT)
+			s_k[q] = ~0ull;
+		__syncthreads();
+		lds_bitonic<NT>(s_k, p2);
+		uint32_t ufl = 0;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const int q = r * NT + t;
+			if (q < (int)nc) {
+				k[r] = s_k[q];
+				if (q == 0 || k[r] != s_k[q - 1])
+					ufl |= 1u << r;
+			}
+		}
+		nu = striped_rank<NT, R>(ufl, c, s_cc);
+#pragma unroll
+		for (int r = 0; r < R; ++r)
+			if (ufl & (1u << r))
+				s_k[c[r]] = k[r];
+		__syncthreads();
+	}
+	select_verify<D, NT>(a, s_k, s_d, nu);
+	select_emit<NT, R>(a, tile, ntiles, s_k, s_d, nu, s_cc, &s_excl);
+}
+
+// the select kernel of a run: the band of its distance (Hamming needs none),
+// NT threads per tile, over fully sorted or grouped candidates
+typedef void (*SelectFn)(SelectArgs);
+template <int NT, bool G>
 inline SelectFn select_kernel_nt(int distance, int measure) {
 	if (measure == 1)
-		return k_select<1, NT>;
+		return G ? k_select_grouped<1, NT> : k_select<1, NT>;
 	switch (distance) {
-	case 1: return k_select<1, NT>;
-	case 2: return k_select<2, NT>;
-	case 3: return k_select<3, NT>;
-	case 4: return k_select<4, NT>;
-	case 5: return k_select<5, NT>;
-	case 6: return k_select<6, NT>;
-	case 7: return k_select<7, NT>;
-	case 8: return k_select<8, NT>;
-	default: return k_select<0, NT>;
+	case 1: return G ? k_select_grouped<1, NT> : k_select<1, NT>;
+	case 2: return G ? k_select_grouped<2, NT> : k_select<2, NT>;
+	case 3: return G ? k_select_grouped<3, NT> : k_select<3, NT>;
+	case 4: return G ? k_select_grouped<4, NT> : k_select<4, NT>;
+	case 5: return G ? k_select_grouped<5, NT> : k_select<5, NT>;
+	case 6: return G ? k_select_grouped<6, NT> : k_select<6, NT>;
+	case 7: return G ? k_select_grouped<7, NT> : k_select<7, NT>;
+	case 8: return G ? k_select_grouped<8, NT> : k_select<8, NT>;
+	default: return G ? k_select_grouped<0, NT> : k_select<0, NT>;
 	}
 }
 
-inline SelectFn select_kernel(int distance, int measure, int nt) {
-	return nt == 512 ? select_kernel_nt<512>(distance, measure) : select_kernel_nt<256>(distance, measure);
+inline SelectFn select_kernel(int distance, int measure, int nt, bool grouped) {
+	if (grouped)
+		return nt == 512 ? select_kernel_nt<512, true>(distance, measure) : select_kernel_nt<256, true>(distance, measure);
+	return nt == 512 ? select_kernel_nt<512, false>(distance, measure) : select_kernel_nt<256, false>(distance, measure);
 }
 
 }  // namespace xtn
