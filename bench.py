@@ -553,33 +553,36 @@ def main():
                                 "(data/gpu_benchmark.csv:42); not the same measurement"},
     }
     eng.configure(profile=False)
-    result["host_boundary"] = host_boundary(eng, chars, offsets)
-    progress("host boundary done")
+
+    def leg(name, done, fn):
+        """a secondary leg: its result under `name`, or the error it raised
+        (recorded in the line, so the main figures are still printed)"""
+        try:
+            result[name] = fn()
+            progress(done)
+        except Exception as e:  # noqa: BLE001 - reported, not hidden
+            result[name] = {"error": f"{type(e).__name__}: {e}"}
+            progress(f"{name} FAILED: {type(e).__name__}: {e}")
+
+    leg("host_boundary", "host boundary done", lambda: host_boundary(eng, chars, offsets))
     if not args.no_batched:
-        result["batched"] = batched_vs_plain(eng, offsets, args.batched_sets, seed)
-        progress("batched vs plain done")
+        leg("batched", "batched vs plain done", lambda: batched_vs_plain(eng, offsets, args.batched_sets, seed))
     eng.close()
     if not args.no_e2e:
-        result["end_to_end_cli"] = end_to_end(chars, offsets, d, measure)
-        progress("CLI end to end done")
+        leg("end_to_end_cli", "CLI end to end done", lambda: end_to_end(chars, offsets, d, measure))
     if not args.no_served:
-        result["served"] = served(args.served_index, args.served_queries, 5)
-        progress("served done")
+        leg("served", "served done", lambda: served(args.served_index, args.served_queries, 5))
     if not args.no_chunked:
-        result["chunked_5M_d1"] = chunked_leg(tune=not args.no_autotune)
-        progress("chunked 5M done")
+        leg("chunked_5M_d1", "chunked 5M done", lambda: chunked_leg(tune=not args.no_autotune))
     if not args.no_distances:
-        result["distances_100k"] = distance_leg()
-        progress("distance scaling done")
+        leg("distances_100k", "distance scaling done", distance_leg)
     if not args.no_overlap:
-        result["overlap"] = overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e, not args.no_autotune)
-        progress("overlap done")
-    result["copy_probe"] = copy_probe()
-    progress("copy probe done")
+        leg("overlap", "overlap done",
+            lambda: overlap_leg(4, 64, 10_000, 500_000, 3, not args.no_e2e, not args.no_autotune))
+    leg("copy_probe", "copy probe done", copy_probe)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(chars, offsets, d, measure, args.cpu_sample,
-                                              args.cpu_threads)
-        progress("CPU baseline done")
+        leg("cpu_baseline", "CPU baseline done",
+            lambda: cpu_baseline(chars, offsets, d, measure, args.cpu_sample, args.cpu_threads))
     print(json.dumps(result))
 
 
