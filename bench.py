@@ -334,7 +334,7 @@ def overlap_leg(seed: int, n_reps: int, size_min: int, size_max: int, steps: int
             r = subprocess.run(cmd, capture_output=True)
             wall.append(time.perf_counter() - t0)
             if r.returncode != 0:
-                raise RuntimeError(f"CLI exit status {r.returncode} ({label}): stderr {r.stderr.decode()[-2000:]!r} "
+                raise RuntimeError(f"CLI exit status {r.returncode} (overlap): stderr {r.stderr.decode()[-2000:]!r} "
                                    f"stdout {r.stdout.decode()[-500:]!r}")
         out["cli_no_output_s"] = min(wall)
         out["cli_seqs_per_s"] = n / min(wall)
