@@ -1495,8 +1495,8 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 	OvMatrix m{h->ov_matrix.as<unsigned long long>(), h->ov_present.as<uint32_t>(), R};
 	{
 		ProfScope ps(h, "ov_exact", (double)n * 16.0);
-		k_exact_term<<<grid_for(U, 256), 256, 0, s>>>(h->ov_ustart.as<uint32_t>(), (int32_t)U, rows, freq_d,
-		                                               rep_d, m);
+		const unsigned g = (unsigned)std::min<int64_t>(grid_for(U, 256), (int64_t)h->n_cu * 3);
+		k_exact_term<<<g, 256, 0, s>>>(h->ov_ustart.as<uint32_t>(), (int32_t)U, rows, freq_d, rep_d, m);
 		HIP_LAUNCH_CHECK(h);
 	}
 	HIP_SYNC(h, s);
@@ -1507,9 +1507,9 @@ int overlap_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callba
 	const int64_t npairs = h->out_len;
 	if (npairs > 0) {
 		ProfScope ps(h, "ov_neighbor", (double)npairs * 40.0);
-		k_neighbor_term<<<grid_for(npairs, 256), 256, 0, s>>>(h->out_pairs.as<int2>(), npairs,
-		                                                       h->ov_ustart.as<uint32_t>(), rows, freq_d,
-		                                                       rep_d, m);
+		const unsigned g = (unsigned)std::min<int64_t>(grid_for(npairs, 256), (int64_t)h->n_cu * 3);
+		k_neighbor_term<<<g, 256, 0, s>>>(h->out_pairs.as<int2>(), npairs, h->ov_ustart.as<uint32_t>(), rows,
+		                                  freq_d, rep_d, m);
 		HIP_LAUNCH_CHECK(h);
 	}
 

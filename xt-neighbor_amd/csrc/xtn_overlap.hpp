@@ -55,10 +55,12 @@ __global__ __launch_bounds__(256) void k_split_info(const int32_t* __restrict__ 
 		top = t2 > top ? t2 : top;
 		bad = b2 < bad ? b2 : bad;
 	}
+	// atomics on one address serialise: a wave only updates a bound it
+	// would move (a stale read costs one atomic more, never a wrong bound)
 	if ((threadIdx.x & (WAVE - 1)) == 0) {
-		if (top)
+		if (top && top > __hip_atomic_load(&ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
 			atomicMax(&ctl[0], top);
-		if (bad != 0xFFFFFFFFu)
+		if (bad != 0xFFFFFFFFu && bad < __hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
 			atomicMin(&ctl[1], bad);
 	}
 }
@@ -201,7 +203,10 @@ __device__ __forceinline__ unsigned long long freq_product(int32_t a, int32_t b)
 	return (unsigned long long)(int64_t)a * (unsigned long long)(int64_t)b;
 }
 
-// exact-match term, one thread per group (kernel.cu:565-584)
+// exact-match term, one thread per group (kernel.cu:565-584), grid-stride
+// over a grid of a few workgroups per CU: each workgroup clears and flushes
+// its LDS matrix once, so the flush's atomics (one per entry it touched,
+// mostly the same diagonal entries in every workgroup) stay few
 __global__ __launch_bounds__(256) void k_exact_term(const uint32_t* __restrict__ gstart, int32_t ngroups,
                                                     const uint32_t* __restrict__ rows,
                                                     const int32_t* __restrict__ freq,
@@ -212,8 +217,7 @@ __global__ __launch_bounds__(256) void k_exact_term(const uint32_t* __restrict__
 	if (lds)
 		ov_lds_init(s_sum, s_present, m.R * m.R);
 	OvAcc acc{s_sum, s_present, lds, m};
-	const int32_t g = blockIdx.x * 256 + threadIdx.x;
-	if (g < ngroups) {
+	for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < ngroups; g += (int64_t)gridDim.x * 256) {
 		const uint32_t b0 = gstart[g], b1 = gstart[g + 1];
 		for (uint32_t a = b0; a < b1; ++a) {
 			const uint32_t ra = rows[a];
@@ -228,7 +232,8 @@ __global__ __launch_bounds__(256) void k_exact_term(const uint32_t* __restrict__
 		ov_lds_flush(s_sum, s_present, m.R * m.R, m);
 }
 
-// neighbour term, one thread per verified pair of groups (kernel.cu:405-435)
+// neighbour term, one thread per verified pair of groups (kernel.cu:405-435),
+// grid-stride as the exact term
 __global__ __launch_bounds__(256) void k_neighbor_term(const int2* __restrict__ pairs, int64_t npairs,
                                                        const uint32_t* __restrict__ gstart,
                                                        const uint32_t* __restrict__ rows,
@@ -240,8 +245,7 @@ __global__ __launch_bounds__(256) void k_neighbor_term(const int2* __restrict__ 
 	if (lds)
 		ov_lds_init(s_sum, s_present, m.R * m.R);
 	OvAcc acc{s_sum, s_present, lds, m};
-	const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
-	if (q < npairs) {
+	for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < npairs; q += (int64_t)gridDim.x * 256) {
 		const int2 p = pairs[q];
 		const uint32_t x0 = gstart[p.x], x1 = gstart[p.x + 1];
 		const uint32_t y0 = gstart[p.y], y1 = gstart[p.y + 1];
