@@ -79,3 +79,31 @@ def test_autotune_and_cache(tmp_path):
         eng.autotune(cache, runs=1)
     with open(cache) as f:
         assert len(json.load(f)["entries"]) == 2
+
+
+@pytest.mark.parametrize("low", [16, 24])
+def test_partial_sort_in_batched_sets_and_rounds(low):
+    # the grouped select over query sets (cross-set candidates dropped) and
+    # over rounds of the smaller id, against the fully sorted tail
+    chars, offsets = xtn.synth_nn(53, 80_000, planted=0.3, max_edits=2)
+    n = len(offsets) - 1
+    so = np.array([0, 5000, 5000, 30000, 61000, n], dtype=np.uint32)
+    want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        base = eng.nn_sets(so, batched=True)
+        eng.set_tuning(sort_low_bits=low)
+        for _ in range(3):  # eager, captured, replayed
+            got = eng.nn_sets(so, batched=True)
+            assert got.keys() == base.keys()
+            for k in base:
+                assert np.array_equal(got[k], base[k]), k
+        n_cand = None
+        for cap in (0, 1 << 15):
+            eng.configure(max_pairs_per_round=cap)
+            m = eng.nn_resident()
+            assert np.array_equal(triples(*eng.copy_nn_output(m)), want)
+            st = eng.stats()
+            n_cand = st["n_candidates"] if n_cand is None else n_cand
+            assert st["n_candidates"] == n_cand
+        assert st["n_rounds"] > 1
