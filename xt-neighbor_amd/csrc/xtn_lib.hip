@@ -126,6 +126,8 @@ struct xtn_handle {
 	void* pin = nullptr;  // pinned upload staging (xtn_set_sequences, a server worker's queries)
 	size_t pin_bytes = 0;
 	hipEvent_t pin_ev[2] = {nullptr, nullptr};
+	void* out_pin = nullptr;  // pinned output staging of xtn_perform (two chunks of pairs and distances)
+	size_t out_pin_bytes = 0;
 	double serve_ratio = 0;  // a server worker's candidates per query record so far
 	bool tail_safe = false;  // a look-back wait hit its limit once: host-checked tail from then on
 	uint32_t spin_limit = LB_SPIN_LIMIT;  // XTN_LOOKBACK_SPIN_LIMIT overrides (tests of the fallback)
@@ -1634,6 +1636,8 @@ void xtn_close(xtn_handle* h) {
 		(void)hipEventDestroy(e);
 	if (h->pin)
 		(void)hipHostFree(h->pin);
+	if (h->out_pin)
+		(void)hipHostFree(h->out_pin);
 	for (hipEvent_t e : h->pin_ev)
 		if (e)
 			(void)hipEventDestroy(e);
@@ -1732,9 +1736,32 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 		return h ? fail(h, XTN_ERR_ARG, "invalid sequence arguments") : XTN_ERR_ARG;
 	if (offsets[0] != 0)
 		return fail(h, XTN_ERR_ARG, "offsets[0] must be 0");
-	// offsets non-decreasing and the prefix lengths, then every residue one
-	// of the 20 (codec.cu:25-38), on host threads; the first bad row is the
-	// one reported
+	HIP_TRY(h, hipSetDevice(h->cfg.device));
+	// Inputs up to SET_PIN_MAX go up from one pinned copy made by the same
+	// host threads that check them: offsets non-decreasing and the prefix
+	// lengths, then every residue one of the 20 (codec.cu:25-38); the first
+	// bad row is the one reported. Nothing reaches the device unless every
+	// check passes, so a rejected input leaves the handle's sequences as they
+	// were. Larger inputs are checked first and then staged in halves.
+	constexpr size_t SET_PIN_MAX = (size_t)64 << 20;
+	const uint32_t total = offsets[n];
+	const size_t cbytes = ((size_t)total + 255) & ~(size_t)255, obytes = sizeof(uint32_t) * ((size_t)n + 1);
+	char* pin = nullptr;
+	if (cbytes + obytes <= SET_PIN_MAX) {
+		if (h->pin_bytes < cbytes + obytes) {
+			if (h->pin)
+				(void)hipHostFree(h->pin);
+			h->pin = nullptr;
+			h->pin_bytes = 0;
+			const size_t want = std::max(cbytes + obytes, (size_t)16 << 20);
+			if (hipHostMalloc(&h->pin, want, hipHostMallocDefault) == hipSuccess)
+				h->pin_bytes = want;
+			else
+				(void)hipGetLastError();
+		}
+		pin = h->pin_bytes >= cbytes + obytes ? static_cast<char*>(h->pin) : nullptr;
+	}
+	uint32_t* pin_offs = pin ? reinterpret_cast<uint32_t*>(pin + cbytes) : nullptr;
 	std::vector<uint8_t> plen((size_t)n);
 	std::vector<int64_t> first_bad(16, INT64_MAX);
 	host_parallel(n, 1 << 16, [&](int t, int64_t lo, int64_t hi) {
@@ -1747,7 +1774,11 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 			const uint32_t len = b - a;
 			plen[(size_t)i] = (uint8_t)(len < 18 ? len : 18);
 		}
+		if (pin_offs)
+			memcpy(pin_offs + lo, offsets + lo, sizeof(uint32_t) * (size_t)(hi - lo + (hi == n ? 1 : 0)));
 	});
+	if (n == 0 && pin_offs)
+		pin_offs[0] = 0;
 	for (int64_t fb : first_bad)
 		if (fb != INT64_MAX) {
 			int64_t row = INT64_MAX;
@@ -1755,19 +1786,29 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 				row = std::min(row, x);
 			return fail(h, XTN_ERR_ARG, "offsets must be non-decreasing (row %lld)", (long long)row);
 		}
-	const uint32_t total = offsets[n];
 	std::vector<char> bad(16, 0);
 	host_parallel((int64_t)total, (int64_t)1 << 20, [&](int t, int64_t lo, int64_t hi) {
-		bad[t] = all_residues(chars + lo, (size_t)(hi - lo)) ? 0 : 1;
+		if (pin) {
+			memcpy(pin + lo, chars + lo, (size_t)(hi - lo));
+			bad[t] = all_residues(pin + lo, (size_t)(hi - lo)) ? 0 : 1;
+		} else {
+			bad[t] = all_residues(chars + lo, (size_t)(hi - lo)) ? 0 : 1;
+		}
 	});
 	for (char x : bad)
 		if (x)
 			return fail(h, XTN_ERROR, "invalid char");
-	HIP_TRY(h, hipSetDevice(h->cfg.device));
 	TRY(ensure(h, h->chars, (size_t)total + 16));
-	TRY(ensure(h, h->offs, sizeof(uint32_t) * ((size_t)n + 1)));
-	TRY(upload_staged(h, h->chars.p, chars, total));
-	TRY(upload_staged(h, h->offs.p, offsets, sizeof(uint32_t) * ((size_t)n + 1)));
+	TRY(ensure(h, h->offs, obytes));
+	if (pin) {
+		if (total)
+			HIP_TRY(h, hipMemcpyAsync(h->chars.p, pin, total, hipMemcpyHostToDevice, h->stream));
+		HIP_TRY(h, hipMemcpyAsync(h->offs.p, pin_offs, obytes, hipMemcpyHostToDevice, h->stream));
+		HIP_SYNC(h, h->stream);
+	} else {
+		TRY(upload_staged(h, h->chars.p, chars, total));
+		TRY(upload_staged(h, h->offs.p, offsets, obytes));
+	}
 	h->n = n;
 	h->n_chars = total;
 	h->out_len = 0;
@@ -1825,17 +1866,31 @@ int xtn_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, 
 		return XTN_OK;
 	}
 	chunk = std::min(chunk, n);
-	// double-buffered pinned staging: chunk k+1 is copied on the side stream
-	// while the callback consumes chunk k
+	// double-buffered pinned staging (kept by the handle from call to call):
+	// chunk k+1 is copied on the side stream while the callback consumes
+	// chunk k
 	xtn_int2* hp[2] = {nullptr, nullptr};
 	int8_t* hd[2] = {nullptr, nullptr};
 	hipEvent_t ready[2] = {nullptr, nullptr};
 	int rc = XTN_OK;
-	for (int b = 0; b < 2 && rc == XTN_OK; ++b) {
-		if (hipHostMalloc((void**)&hp[b], sizeof(xtn_int2) * (size_t)chunk, hipHostMallocDefault) != hipSuccess ||
-		    hipHostMalloc((void**)&hd[b], (size_t)chunk, hipHostMallocDefault) != hipSuccess ||
-		    hipEventCreateWithFlags(&ready[b], hipEventDisableTiming) != hipSuccess)
+	const size_t pbytes = (sizeof(xtn_int2) * (size_t)chunk + 255) & ~(size_t)255;
+	const size_t dbytes = ((size_t)chunk + 255) & ~(size_t)255;
+	if (h->out_pin_bytes < 2 * (pbytes + dbytes)) {
+		if (h->out_pin)
+			(void)hipHostFree(h->out_pin);
+		h->out_pin = nullptr;
+		h->out_pin_bytes = 0;
+		if (hipHostMalloc(&h->out_pin, 2 * (pbytes + dbytes), hipHostMallocDefault) == hipSuccess)
+			h->out_pin_bytes = 2 * (pbytes + dbytes);
+		else
 			rc = fail(h, XTN_ERR_NOMEM, "pinned staging allocation failed");
+	}
+	for (int b = 0; b < 2 && rc == XTN_OK; ++b) {
+		char* base = static_cast<char*>(h->out_pin) + (size_t)b * (pbytes + dbytes);
+		hp[b] = reinterpret_cast<xtn_int2*>(base);
+		hd[b] = reinterpret_cast<int8_t*>(base + pbytes);
+		if (hipEventCreateWithFlags(&ready[b], hipEventDisableTiming) != hipSuccess)
+			rc = fail(h, XTN_ERR_HIP, "event creation failed");
 	}
 	auto issue = [&](int64_t start, int b) -> int {
 		const int64_t len = std::min(chunk, n - start);
@@ -1868,14 +1923,9 @@ int xtn_perform(xtn_handle* h, const xtn_seq_info* info, xtn_callback callback, 
 			callback(user, &out);
 	}
 	(void)hipStreamSynchronize(h->copy_stream);
-	for (int b = 0; b < 2; ++b) {
-		if (hp[b])
-			(void)hipHostFree(hp[b]);
-		if (hd[b])
-			(void)hipHostFree(hd[b]);
+	for (int b = 0; b < 2; ++b)
 		if (ready[b])
 			(void)hipEventDestroy(ready[b]);
-	}
 	h->stats.ms_output = now_ms() - t0;
 	return rc;
 }
