@@ -236,6 +236,31 @@ def test_invalid_residue_rejected(engine):
     assert engine.nn()[0].shape == (0, 2)
 
 
+def test_rejected_input_keeps_previous_sequences(engine):
+    # a rejected xtn_set_sequences (bad residue past the first MiB, bad
+    # offsets) reaches nothing on the device: the sequences set before are
+    # still searched, with the same output
+    # (the module's engine keeps the configuration of the test before)
+    engine.configure(distance=1, measure=0, max_records_per_chunk=0, output_chunk_pairs=0, bucket_capacity=0)
+    chars, offsets = xtn.synth_nn(61, 150_000, planted=0.3, max_edits=1)
+    engine.set_sequences(chars=chars, offsets=offsets)
+    p0, d0, _ = engine.nn()
+    assert len(d0) > 0
+    other_c, other_o = xtn.synth_nn(62, 120_000, planted=0.3, max_edits=1)
+    bad_c = other_c.copy()
+    bad_c[(1 << 20) + 5] = ord("X")
+    with pytest.raises(xtn.XtnError) as e:
+        engine.set_sequences(chars=bad_c, offsets=other_o)
+    assert e.value.status == 1
+    bad_o = other_o.copy()
+    bad_o[1000] = bad_o[1001] + 1
+    with pytest.raises(xtn.XtnError) as e:
+        engine.set_sequences(chars=other_c, offsets=bad_o)
+    assert e.value.status == 3
+    p1, d1, _ = engine.nn()
+    assert np.array_equal(p1, p0) and np.array_equal(d1, d0)
+
+
 def test_cli_end_to_end(tmp_path):
     env = dict(os.environ, LC_ALL="C")
     for d, measure, fx in [("1", "leven", "expected_dummy_d1.txt"), ("1", "hamming", "expected_dummy_d1.txt"),
