@@ -185,6 +185,25 @@ def test_100k_d2_vs_port(engine, measure):
     assert np.array_equal(got, port)
 
 
+@pytest.mark.parametrize("measure", [0, 1])
+def test_100k_d3_vs_port(engine, measure):
+    # the table form of distance 3 (three deleted slots, G^-2 term) over
+    # thousands of generator tiles
+    chars, offsets = xtn.synth_nn(5, 100000, planted=0.25, max_edits=3)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=3, measure=measure)
+    port, _ = oracle.nn_port_mt(chars, offsets, 3, measure, min(16, os.cpu_count() or 1))
+    assert np.array_equal(got, port.astype(np.int64))
+
+
+@pytest.mark.parametrize("measure", [0, 1])
+def test_20k_d4_vs_port(engine, measure):
+    # distance 4: variants unranked in colex order, keyed by the word hash
+    chars, offsets = xtn.synth_nn(6, 20000, planted=0.3, max_edits=4)
+    got, _ = run(engine, chars=chars, offsets=offsets, d=4, measure=measure)
+    port, _ = oracle.nn_port_mt(chars, offsets, 4, measure, min(16, os.cpu_count() or 1))
+    assert np.array_equal(got, port.astype(np.int64))
+
+
 def test_shuffle_invariance(engine):
     """01_correctness.ipynb cell 18: permute the input, map indexes back"""
     chars, offsets = xtn.synth_nn(9, 50000, planted=0.25, max_edits=2)
