@@ -417,6 +417,10 @@ def distance_leg(steps: int = 5) -> dict:
             ms, st = timed_runs(eng, steps)
             out[f"d{d}"] = {"ms": ms, "seqs_per_s": n / (ms / 1e3), "variants": st["n_variants"],
                             "unique_pairs": st["n_unique"], "output_pairs": st["n_output"]}
+    out["reference_published_s"] = {"d1": 0.436, "d2": 0.836, "d3": 2.676, "d4": 11.03, "d5": 49.10,
+                                    "note": "XTNeighbor-streaming (V2) at 100k, end to end incl. process "
+                                            "start, data/gpu_dist_benchmark.csv:2-13; not the same "
+                                            "measurement"}
     return out
 
 
