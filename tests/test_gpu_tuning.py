@@ -107,3 +107,23 @@ def test_partial_sort_in_batched_sets_and_rounds(low):
             n_cand = st["n_candidates"] if n_cand is None else n_cand
             assert st["n_candidates"] == n_cand
         assert st["n_rounds"] > 1
+
+
+def test_chunks_cut_by_bucket_target():
+    # 37M records over a 512-record bucket target exceed the 65536 buckets a
+    # chunk may hold, so the run splits into chunks by bucket size: the
+    # records are generated once and each chunk copies its part; a chunk size
+    # set by the caller generates every part on its own. Both match the oracle,
+    # over one round and over several.
+    chars, offsets = xtn.synth_nn(54, 300_000, planted=0.25, max_edits=2)
+    want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        eng.set_tuning(bucket_capacity=2048, bucket_target=512)
+        for chunk, pairs in ((0, 0), (0, 1 << 16), (12_000_000, 0)):
+            eng.configure(max_records_per_chunk=chunk, max_pairs_per_round=pairs)
+            n = eng.nn_resident()
+            assert np.array_equal(triples(*eng.copy_nn_output(n)), want), (chunk, pairs)
+            st = eng.stats()
+            assert st["n_chunks"] > 1
+            assert (st["n_rounds"] > 1) == (pairs > 0)

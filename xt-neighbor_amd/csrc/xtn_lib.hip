@@ -78,6 +78,7 @@ struct xtn_handle {
 
 	// workspaces (grow-only)
 	DBuf bcount, ctl;  // region layout: bucket cursors, coarse bucket/tile table
+	DBuf recs_all, recs_part;  // chunked runs: every record once, then ordered by part
 	DBuf pk, voff, recs, recs_alt, hist, scan_tmp, runs_small, runs_big, big_pre, pairs, pairs_alt,
 	    flags, pos, dist, out_pairs, out_dist, ctrs, part_hist, scalar;
 	// overlap-mode workspaces
@@ -1057,8 +1058,58 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	const unsigned gen_grid = grid_for((int64_t)V, GEN_TILE);
 
 	std::vector<unsigned long long> part_count(nparts, 0);
+	std::vector<int64_t> part_start(nparts + 1, 0);
 	int64_t cap_recs = (int64_t)V;
-	if (nparts > 1) {
+	// Chunks cut by bucket size rather than by memory (HBM holds two full
+	// record buffers besides a chunk's working set): every record is
+	// generated once, ordered by its top 8 hash bits (the part bits lead
+	// them), and each chunk copies its part's slice instead of generating
+	// all V records again to keep its own
+	const bool gen_once = nparts > 1 && pbits <= 8 && h->cfg.max_records_per_chunk <= 0 && !h->no_alloc &&
+	                      (double)V * 16.0 + (double)(V >> pbits) * 1.25 * 40.0 <= (double)free_b * 0.6;
+	if (gen_once) {
+		TRY(prologue());
+		const int64_t vt = ((int64_t)V + RS_TILE - 1) / RS_TILE;
+		TRY(ensure(h, h->recs_all, sizeof(uint64_t) * (size_t)V));
+		TRY(ensure(h, h->recs_part, sizeof(uint64_t) * (size_t)V));
+		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * vt)));
+		uint32_t* hist = h->hist.as<uint32_t>();
+		ga.out = h->recs_all.as<uint64_t>();
+		ga.cap = (int64_t)V;
+		ga.digit_hist = hist;
+		ga.ntiles = vt;
+		ga.digit_shift = 56;
+		{
+			StageScope sg(h, "generate");
+			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + 1024.0 * (double)vt);
+			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		}
+		ga.digit_hist = nullptr;
+		uint32_t* dtot = nullptr;
+		TRY(digit_scan(h, hist, vt, &dtot));
+		{
+			StageScope sg(h, "sort");
+			ProfScope ps(h, "part_scatter", 16.0 * (double)V);
+			k_radix_scatter<false, RS_NT, true><<<(unsigned)vt, RS_NT, 0, s>>>(
+			    h->recs_all.as<uint64_t>(), nullptr, h->recs_part.as<uint64_t>(), nullptr, (int64_t)V, 56, hist, vt);
+			HIP_LAUNCH_CHECK(h);
+		}
+		uint32_t tot[RS_BINS];
+		HIP_TRY(h, hipMemcpyAsync(tot, dtot, sizeof(tot), hipMemcpyDeviceToHost, s));
+		HIP_SYNC(h, s);
+		const int per = RS_BINS >> pbits;  // top-8-bit digits per part
+		for (int p = 0; p < nparts; ++p) {
+			unsigned long long c = 0;
+			for (int k = 0; k < per; ++k)
+				c += tot[p * per + k];
+			part_count[p] = c;
+			part_start[p + 1] = part_start[p] + (int64_t)c;
+		}
+		cap_recs = 0;
+		for (auto c : part_count)
+			cap_recs = std::max<int64_t>(cap_recs, (int64_t)c);
+	} else if (nparts > 1) {
 		TRY(prologue());
 		TRY(ensure(h, h->part_hist, sizeof(unsigned long long) * nparts));
 		k_fill_u64<<<grid_for(nparts, 256), 256, 0, s>>>(h->part_hist.as<unsigned long long>(), nparts, 0ull);
@@ -1149,6 +1200,11 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (g.fused_hist ? 1024.0 * g.rtiles : 0.0));
 			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
 			HIP_LAUNCH_CHECK(h);
+		} else if (gen_once) {
+			ProfScope ps(h, "part_copy", 16.0 * (double)part_count[part]);
+			if (part_count[part])
+				HIP_TRY(h, hipMemcpyAsync(h->recs.p, h->recs_part.as<uint64_t>() + part_start[part],
+				                          sizeof(uint64_t) * (size_t)part_count[part], hipMemcpyDeviceToDevice, s));
 		} else {
 			ga.part_id = (uint32_t)part;
 			ga.append = ctr_field(h, offsetof(RunCtrs, append));
@@ -1619,7 +1675,8 @@ void xtn_close(xtn_handle* h) {
 		return;
 	(void)hipSetDevice(h->cfg.device);
 	(void)hipStreamSynchronize(h->stream);
-	DBuf* bufs[] = {&h->bcount, &h->ctl, &h->chars, &h->offs, &h->pk, &h->voff, &h->recs, &h->recs_alt, &h->hist,
+	DBuf* bufs[] = {&h->recs_all, &h->recs_part, &h->bcount, &h->ctl, &h->chars, &h->offs, &h->pk, &h->voff,
+	                &h->recs, &h->recs_alt, &h->hist,
 	                &h->scan_tmp, &h->runs_small, &h->runs_big, &h->big_pre, &h->pairs,
 	                &h->pairs_alt, &h->flags, &h->pos, &h->dist, &h->out_pairs, &h->out_dist,
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
