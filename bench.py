@@ -438,6 +438,26 @@ def progress(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def coarse_ab(eng, steps):
+    """ms per step of the same workload with the records' first ordering
+    pass in the generator and as a pass of its own, back to back"""
+    out = {}
+    for sep in (0, 1):
+        eng.configure(coarse_separate=sep)
+        for _ in range(3):
+            eng.nn_resident()
+        hip_sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.nn_resident()
+        hip_sync()
+        key = "separate" if sep else "in_generator"
+        out[key + "_ms"] = (time.perf_counter() - t0) / steps * 1e3
+        out[key + "_took_generator_path"] = bool(eng.stats()["coarse_in_generator"])
+    eng.configure(coarse_separate=0)
+    return out
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE
     runs summarised in profiles/ (calibrated on a copy kernel), or None"""
@@ -468,6 +488,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default engine parameters")
     ap.add_argument("--no-distances", action="store_true")
     ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--no-coarse-ab", action="store_true")
+    ap.add_argument("--coarse-separate", action="store_true",
+                    help="run the records' first ordering pass as a pass of its own (not in the generator)")
     ap.add_argument("--served-index", type=int, default=10_000_000)
     ap.add_argument("--served-queries", type=int, default=64)
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
@@ -490,6 +513,7 @@ def main():
     chars, offsets = xtn.synth_nn(seed, n, mu, sigma, lmin, lmax, planted, edits)
 
     eng = xtn.Engine(distance=d, measure=measure, profile=False)
+    eng.configure(coarse_separate=1 if args.coarse_separate else 0)
     eng.set_sequences(chars=chars, offsets=offsets)
     tuning = autotuned(eng, not args.no_autotune)
     for _ in range(args.warmup):
@@ -520,7 +544,8 @@ def main():
     hip_sync()
     profiled_ms = (time.perf_counter() - tp0) / args.steps * 1e3
     progress(f"{args.workload}: {step_s * 1e3:.3f} ms per step")
-    kernels = ["gen", "sort_rec_hist", "sort_rec_scatter", "region", "bucket", "runs", "pairs_small", "pairs_big",
+    kernels = ["gen", "gen_coarse", "sort_rec_hist", "sort_rec_scatter", "region", "bucket", "runs", "pairs_small",
+               "pairs_big",
                "sort_seg_hist", "sort_seg_scatter", "tail_hist", "sort_pairs", "select", "pack", "scan",
                "gen_count"]
     prof = {k: eng.kernel_profile(k) for k in kernels}
@@ -553,6 +578,7 @@ def main():
         "syncs_per_step": st["n_syncs"],
         "graph_replay": bool(st["graph_replays"]),
         "tail_sort_passes": st["tail_passes"],
+        "coarse_in_generator": bool(st["coarse_in_generator"]),
         "output_pairs": out_len,
         "variants": st["n_variants"],
         "candidate_pairs": st["n_candidates"],
@@ -581,6 +607,8 @@ def main():
             result[name] = {"error": f"{type(e).__name__}: {e}"}
             progress(f"{name} FAILED: {type(e).__name__}: {e}")
 
+    if not args.no_coarse_ab:
+        leg("coarse_pass", "coarse pass A/B done", lambda: coarse_ab(eng, args.steps))
     leg("host_boundary", "host boundary done", lambda: host_boundary(eng, chars, offsets))
     if not args.no_batched:
         leg("batched", "batched vs plain done", lambda: batched_vs_plain(eng, offsets, args.batched_sets, seed))

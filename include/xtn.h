@@ -90,7 +90,10 @@ typedef struct {
 	                                  HIP graph (0: a single-chunk run is
 	                                  captured once it repeats and replayed
 	                                  while it keeps its sizes) */
-	int32_t reserved;
+	int32_t coarse_separate;       /* 1: the records' first ordering pass (by
+	                                  their top 8 hash bits) runs as a pass
+	                                  of its own; 0: the generator does it
+	                                  when one chunk uses bucket regions */
 	int64_t max_pairs_per_round;   /* 0: candidate pairs per round sized from
 	                                  free HBM (at most 2^30 - 1); > 0: cap
 	                                  them. A search with more candidates runs
@@ -124,6 +127,11 @@ typedef struct {
 	                          the low bits of each group in LDS, all of them
 	                          after a group outgrew the select's window on
 	                          this handle */
+	int64_t coarse_in_generator; /* 1: the generator ordered the records by their
+	                                top 8 hash bits itself; 0: a separate pass did
+	                                (always after a sub-range of the generator's
+	                                pass overflowed on this handle, until new
+	                                sequences) */
 } xtn_stats;
 
 /* fills *cfg with defaults: distance 1, Levenshtein, device 0 */

@@ -1,0 +1,78 @@
+"""The records' first ordering pass (by their top 8 hash bits) done by the
+generator itself, on the GPU: it gives exactly the separate pass's output and
+the oracle's, for both measures and several distances; a sub-range that
+overflows (many copies of one string in one generator tile) sends the run back
+to the separate pass, which the handle keeps until new sequences; and a bucket
+that outgrows its region after the generator's pass is regrouped exactly from
+the compacted sub-ranges."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import xtn
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def triples(p, d):
+    return np.concatenate([p.astype(np.int64), d.astype(np.int64)[:, None]], axis=1)
+
+
+def run(eng, **cfg):
+    eng.configure(**cfg)
+    n = eng.nn_resident()
+    return triples(*eng.copy_nn_output(n)), eng.stats()
+
+
+@pytest.mark.parametrize("d,measure,n", [(1, 0, 45_000), (1, 1, 45_000), (2, 0, 60_000), (2, 1, 60_000),
+                                         (3, 0, 40_000)])
+def test_generator_pass_equals_separate_pass(d, measure, n):
+    chars, offsets = xtn.synth_nn(61 + d, n, planted=0.3, max_edits=d)
+    want = oracle.nn_port_mt(chars, offsets, d, measure, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=d) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        for sep in (0, 1, 0):
+            for _ in range(3):  # eager, captured, replayed
+                got, st = run(eng, distance=d, measure=measure, coarse_separate=sep)
+                assert np.array_equal(got, want), (sep, st)
+                assert st["coarse_in_generator"] == 1 - sep
+
+
+def test_overflowing_sub_range_falls_back():
+    # 455 copies of an 8-letter string fill the first generator tile (9
+    # records each at d=1): each of its 9 variants puts 455 records into one
+    # sub-range whose capacity is ~165 against a mean of ~44
+    base = xtn.decode(*xtn.synth_nn(64, 45_000, planted=0.3, max_edits=1))
+    seqs = ["CDEFGHIK"] * 455 + base
+    chars, offsets = xtn.encode(seqs)
+    want = oracle.nn_port_mt(chars, offsets, 1, 0, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=1) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        for _ in range(3):
+            got, st = run(eng, distance=1, measure=0, coarse_separate=0)
+            assert np.array_equal(got, want)
+            assert st["coarse_in_generator"] == 0
+        # new sequences without the copies: the generator's pass again
+        chars2, offsets2 = xtn.encode(base)
+        want2 = oracle.nn_port_mt(chars2, offsets2, 1, 0, THREADS)[0].astype(np.int64)
+        eng.set_sequences(chars=chars2, offsets=offsets2)
+        got, st = run(eng, distance=1, measure=0, coarse_separate=0)
+        assert np.array_equal(got, want2)
+        assert st["coarse_in_generator"] == 1
+
+
+@pytest.mark.parametrize("cap", [75, 1900])
+def test_region_overflow_after_generator_pass(cap):
+    # buckets above `cap` records outgrow their regions: the chunk is
+    # regrouped on the exact sort from the compacted sub-ranges
+    chars, offsets = xtn.synth_nn(65, 60_000, planted=0.3, max_edits=2)
+    want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        got, st = run(eng, distance=2, measure=0, bucket_capacity=cap, coarse_separate=0)
+        assert np.array_equal(got, want)
+        assert st["coarse_in_generator"] == 1

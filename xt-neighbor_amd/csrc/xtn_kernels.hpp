@@ -44,7 +44,7 @@ struct RunCtrs {
 	unsigned long long n_cand;       // candidate pairs written (append cursor)
 	unsigned long long n_unique;     // unique candidate pairs verified
 	unsigned long long append;       // partition append cursor
-	unsigned long long pad;
+	unsigned long long coarse_ovf;   // the generator's coarse pass lost records (sub-range full)
 };
 
 struct Run {
@@ -380,7 +380,12 @@ constexpr int GEN_NT = 256;
 constexpr int GEN_IT = 16;  // GEN_TILE == RS_TILE: a generator tile is a radix tile
 constexpr int GEN_TILE = GEN_NT * GEN_IT;
 
-enum GenMode { GEN_DIRECT = 0, GEN_HIST = 1, GEN_APPEND = 2 };
+enum GenMode { GEN_DIRECT = 0, GEN_HIST = 1, GEN_APPEND = 2, GEN_COARSE = 3 };
+// threads of the generator's coarse pass: 512 holds the tile's sequences
+// in a workgroup for half as many records per lane, and the tile's set-up
+// (sequence lookups and hash tables, a chain of dependent loads) then weighs
+// twice as much; measured 4% slower at 1M CDR3s, d=2
+constexpr int GEN_COARSE_NT = 256;
 
 struct GenArgs {
 	const uint64_t* pk;
@@ -401,7 +406,11 @@ struct GenArgs {
 	uint32_t* digit_hist;           // GEN_DIRECT, nullable: per-tile counts of the
 	int64_t ntiles;                 // 8-bit digit at digit_shift, one row of
 	int32_t digit_shift;            // RS_BINS per tile as k_radix_hist writes them
-	int32_t pad2;
+	                                // (GEN_COARSE: the coarse digit)
+	int32_t sub_bits;               // GEN_COARSE: 2^sub_bits sub-ranges per digit
+	uint32_t* cursor;               // GEN_COARSE: slots taken per (digit, sub-range)
+	unsigned long long* coarse_ovf; // GEN_COARSE: set when a sub-range outgrew sub_cap
+	int64_t sub_cap;                // GEN_COARSE: slots per sub-range
 };
 
 __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n, uint64_t x) {
@@ -462,8 +471,9 @@ __device__ __forceinline__ void seq_hash_table(const GenArgs& a, int64_t sid, ui
 	tab[MAX_PREFIX + 1] = c;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
+template <int MODE, int NT = GEN_NT>
+__global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
+	constexpr int IT = GEN_TILE / NT;  // records per lane
 	// variant offsets of the tile's sequences minus r0, clamped to
 	// [0, GEN_TILE]; the first one (<= 0) is kept whole in s_v0. A tile of
 	// at most GEN_TBL sequences uses only the head of the offsets and keeps
@@ -479,6 +489,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			uint16_t head[GEN_PKL + 8];
 			uint64_t pk[GEN_PKL][2];
 		} p;
+		uint64_t keys[MODE == GEN_COARSE ? GEN_TILE / 2 : 1];  // GEN_COARSE, after the loop: half the sorted tile
 	} s_v;
 	uint16_t* const s_vrel = s_v.vrel;
 	__shared__ int32_t s_v0;
@@ -490,7 +501,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
 	__shared__ uint32_t s_hist[MODE == GEN_HIST ? 4096 : 1];
-	__shared__ uint32_t s_dig[MODE == GEN_DIRECT ? RS_BINS : 1];
+	__shared__ uint32_t s_dig[MODE == GEN_DIRECT ? RS_BINS : MODE == GEN_COARSE ? RS_BINS + 1 : 1];
 
 	const int t = threadIdx.x;
 	const int64_t r0 = (int64_t)blockIdx.x * GEN_TILE;
@@ -502,18 +513,18 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	if (small_d) {
 		const uint32_t* src = a.distance == 1 ? c_del_tables.d1 : a.distance == 2 ? c_del_tables.d2 : c_del_tables.d3;
 		const int ns = a.distance == 1 ? DEL_SETS1 : a.distance == 2 ? DEL_SETS2 : DEL_SETS3;
-		for (int i = t; i < ns; i += GEN_NT)
+		for (int i = t; i < ns; i += NT)
 			s_u.del[i] = src[i];
 	} else {
-		load_gen_tables(&s_u.tab, t, GEN_NT);
+		load_gen_tables(&s_u.tab, t, NT);
 	}
-	init_slot_masks(s_hm, s_lm, t, GEN_NT);
-	if (MODE == GEN_DIRECT)
-		for (int i = t; i < RS_BINS; i += GEN_NT)
+	init_slot_masks(s_hm, s_lm, t, NT);
+	if (MODE == GEN_DIRECT || MODE == GEN_COARSE)
+		for (int i = t; i < (MODE == GEN_COARSE ? RS_BINS + 1 : RS_BINS); i += NT)
 			s_dig[i] = 0;
 	if (MODE == GEN_HIST) {
 		const int nb = 1 << a.part_bits;
-		for (int i = t; i < nb; i += GEN_NT)
+		for (int i = t; i < nb; i += NT)
 			s_hist[i] = 0;
 	}
 	if (t == 0) {
@@ -541,7 +552,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	const int64_t slo = s_lo;
 	const int cnt = s_cnt;  // sequences touching this tile, <= GEN_TILE
 	// a sequence has at most 2^18 variants: the first offset fits 32 bits
-	for (int i = t; i <= cnt; i += GEN_NT) {
+	for (int i = t; i <= cnt; i += NT) {
 		const int64_t rel = (int64_t)a.voff[slo + i] - r0;
 		s_vrel[i] = (uint16_t)(rel < 0 ? 0 : (rel > GEN_TILE ? GEN_TILE : rel));
 		if (i == 0)
@@ -553,10 +564,10 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	const bool poly = poly_keys(a.distance);
 	const int form = poly && cnt <= GEN_TBL ? 0 : cnt <= GEN_PKL ? 1 : 2;
 	if (form == 0)
-		for (int i = t; i < cnt; i += GEN_NT)
+		for (int i = t; i < cnt; i += NT)
 			seq_hash_table(a, slo + i, s_v.t.tab[i]);
 	else if (form == 1)
-		for (int i = t; i < cnt; i += GEN_NT) {
+		for (int i = t; i < cnt; i += NT) {
 			s_v.p.pk[i][0] = a.pk[2 * (slo + i)];
 			s_v.p.pk[i][1] = a.pk[2 * (slo + i) + 1];
 		}
@@ -564,7 +575,7 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	const int v0 = s_v0;
 
 	const uint64_t idmask = (1ull << a.idb) - 1ull;
-	uint64_t kept[MODE == GEN_APPEND ? GEN_IT : 1];
+	uint64_t kept[MODE == GEN_APPEND || MODE == GEN_COARSE ? IT : 1];
 	uint32_t nkept = 0;
 	// The lane's first record: the sequence holding it (largest idx with
 	// s_vrel[idx] <= t). Every iteration then finds the next record's
@@ -596,16 +607,16 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 		// kept rolled: the body holds the whole unranking, sixteen copies of
 		// it would not fit the instruction cache
 #pragma unroll 1
-		for (int it = 0; it < GEN_IT; ++it) {
-			const int rr = it * GEN_NT + t;  // record index within the tile
+		for (int it = 0; it < IT; ++it) {
+			const int rr = it * NT + t;  // record index within the tile
 			const int64_t r = r0 + rr;
 			const bool valid = r < r1;
 			const int idx = idx_n, start = start_n;
 			const uint64_t lo64 = lo_n, hiw = hi_n;
-			if (r + GEN_NT < r1) {  // records of a lane only move forward
+			if (r + NT < r1) {  // records of a lane only move forward
 				// the next four sequence starts per look (the offsets do not
 				// decrease; indexes past cnt count as "beyond")
-				const int rn = rr + GEN_NT;
+				const int rn = rr + NT;
 				int j = idx, st = start;
 				for (;;) {
 					int sv[4];
@@ -688,10 +699,15 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 			} else if (MODE == GEN_HIST) {
 				if (valid)
 					atomicAdd(&s_hist[a.part_bits ? (rec >> (64 - a.part_bits)) : 0], 1u);
+			} else if (MODE == GEN_COARSE) {
+				// the record stays in a register; lanes past the end count
+				// into the spare bin s_dig[RS_BINS]
+				kept[it] = rec;
+				atomicAdd(&s_dig[valid ? (uint32_t)(rec >> a.digit_shift) & 0xFFu : (uint32_t)RS_BINS], 1u);
 			} else {
 				const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
 				if (mine)
-					kept[nkept++ & (GEN_IT - 1)] = rec;
+					kept[nkept++ & (IT - 1)] = rec;
 			}
 		}
 	};
@@ -702,24 +718,88 @@ __global__ __launch_bounds__(GEN_NT) void k_gen(GenArgs a) {
 	else
 		gen_loop(std::integral_constant<int, 2>{});
 	if (MODE == GEN_APPEND) {
-		__shared__ uint32_t s_tmp[GEN_NT / WAVE + 1];
+		__shared__ uint32_t s_tmp[NT / WAVE + 1];
 		__shared__ unsigned long long s_base;
-		unsigned long long slot = block_append<GEN_NT>(a.append, nkept, s_tmp, &s_base);
+		unsigned long long slot = block_append<NT>(a.append, nkept, s_tmp, &s_base);
 		for (uint32_t i = 0; i < nkept; ++i, ++slot)
 			if ((int64_t)slot < a.cap)
 				a.out[slot] = kept[i];
 	}
 	if (MODE == GEN_DIRECT && a.digit_hist) {
 		__syncthreads();
-		for (int i = t; i < RS_BINS; i += GEN_NT)
+		for (int i = t; i < RS_BINS; i += NT)
 			a.digit_hist[(int64_t)blockIdx.x * RS_BINS + i] = s_dig[i];
 	}
 	if (MODE == GEN_HIST) {
 		__syncthreads();
 		const int nb = 1 << a.part_bits;
-		for (int i = t; i < nb; i += GEN_NT)
+		for (int i = t; i < nb; i += NT)
 			if (s_hist[i])
 				atomicAdd(&a.part_hist[i], (unsigned long long)s_hist[i]);
+	}
+	if constexpr (MODE == GEN_COARSE) {
+		// The tile's records go to their coarse digit's sub-range (the tile
+		// index mod 2^sub_bits; tiles of one sub-range run on one XCD, so the
+		// runs of neighbouring tiles meet in its L2): one slot claim per
+		// digit, a position in the tile sorted by digit from one LDS atomic
+		// per record (slot order inside a sub-range is free), then the
+		// records written out in digit runs. The generator's tables are dead
+		// after the loop: the sorted tile takes their place, and the slot
+		// claims are in flight while it is placed.
+		__syncthreads();
+		uint32_t* const s_dpre = reinterpret_cast<uint32_t*>(&s_u);
+		uint32_t* const s_cur = s_dpre + RS_BINS;
+		uint32_t* const s_gbase = s_cur + RS_BINS + 1;
+		uint64_t* const s_keys = s_v.keys;
+		__shared__ uint32_t s_tmp[NT / WAVE + 1];
+		static_assert(sizeof(s_u) >= (3 * RS_BINS + 1) * sizeof(uint32_t), "offsets fit the generator's tables");
+		const uint32_t c = t < RS_BINS ? s_dig[t] : 0u;
+		const uint32_t ex = block_excl_scan<NT>(c, s_tmp, (uint32_t*)nullptr);
+		const uint32_t sub = blockIdx.x & ((1u << a.sub_bits) - 1u);
+		if (t < RS_BINS) {
+			s_dpre[t] = ex;
+			s_cur[t] = ex;
+		}
+		if (t == 0)
+			s_cur[RS_BINS] = GEN_TILE;  // lanes past the end: positions at or beyond the tile
+		__syncthreads();
+		uint32_t b = 0;
+		if (t < RS_BINS && c)
+			b = atomicAdd(&a.cursor[((uint32_t)t << a.sub_bits) | sub], c);
+		uint32_t rank[IT];  // the record's position in the tile sorted by digit
+#pragma unroll
+		for (int r = 0; r < IT; ++r) {
+			const bool valid = r0 + r * NT + t < r1;
+			rank[r] = atomicAdd(&s_cur[valid ? (uint32_t)(kept[r] >> a.digit_shift) & 0xFFu : (uint32_t)RS_BINS], 1u);
+		}
+		// the sorted tile in two halves, so that LDS holds half of it (more
+		// workgroups per CU for the generation): a record goes to the half
+		// its position falls in; the digit runs keep their length
+		const int tile_n = (int)(r1 - r0);
+		constexpr uint32_t HALF = GEN_TILE / 2;
+		for (uint32_t lo_pos = 0; lo_pos < (uint32_t)GEN_TILE; lo_pos += HALF) {
+#pragma unroll
+			for (int r = 0; r < IT; ++r) {
+				const uint32_t pos = rank[r] - lo_pos;
+				if (pos < HALF)  // also false below lo_pos (unsigned) and for lanes past the end
+					s_keys[pos] = kept[r];
+			}
+			if (lo_pos == 0 && t < RS_BINS) {
+				if (c && (int64_t)b + c > a.sub_cap)
+					atomicOr(a.coarse_ovf, 1ull);  // records lost: the host redoes the run
+				s_gbase[t] = b;
+			}
+			__syncthreads();
+			const int n_half = tile_n - (int)lo_pos < (int)HALF ? tile_n - (int)lo_pos : (int)HALF;
+			for (int i = t; i < n_half; i += NT) {
+				const uint64_t key = s_keys[i];
+				const uint32_t dg = (uint32_t)(key >> a.digit_shift) & 0xFFu;
+				const int64_t slot = (int64_t)s_gbase[dg] + ((int)lo_pos + i - (int)s_dpre[dg]);
+				if (slot < a.sub_cap)
+					a.out[(int64_t)((dg << a.sub_bits) | sub) * a.sub_cap + slot] = key;
+			}
+			__syncthreads();  // the half is written before the next one is placed
+		}
 	}
 }
 
@@ -925,17 +1005,163 @@ __global__ __launch_bounds__(RS_BINS) void k_coarse_tiles(const uint32_t* __rest
 	}
 }
 
+// The coarse pass fused into the generator (k_gen<GEN_COARSE>) leaves the
+// records in sub-ranges: sub-range j of coarse digit c, segment
+// e = (c << sub_bits) | j, holds min(cursor[e], sub_cap) records from
+// e * sub_cap. The passes after it read a coarse digit's sub-ranges as one
+// run of records (position p of digit c lies in the sub-range j with
+// segpre[c][j] <= p < segpre[c][j + 1]), cut into tiles as the dense layout
+// would be. k_seg_tiles writes, in `seg`: the first tile of every coarse
+// digit (SEG_TSTART, 257 words, as k_coarse_tiles does), the digit's first
+// record in a dense layout (SEG_CSTART, 257 words) and the prefix counts of
+// its sub-ranges (SEG_PRE, S + 1 words per digit).
+constexpr int SEG_NT = 1024;
+constexpr int SEG_MAX_SUB = 64;  // sub-ranges per coarse digit, at most
+constexpr int SEG_TSTART = 0;
+constexpr int SEG_CSTART = RS_BINS + 1;
+constexpr int SEG_PRE = 2 * (RS_BINS + 1);
+constexpr int seg_words(int nsub) { return SEG_PRE + RS_BINS * (nsub + 1); }
+
+__global__ __launch_bounds__(SEG_NT) void k_seg_tiles(const uint32_t* __restrict__ cursor, int sub_bits,
+                                                      int64_t sub_cap, uint32_t* __restrict__ seg) {
+	constexpr int PER_MAX = RS_BINS * SEG_MAX_SUB / SEG_NT;  // segments per thread, at most
+	__shared__ uint32_t s_part[SEG_NT];
+	__shared__ uint32_t s_tot[RS_BINS];
+	__shared__ uint32_t s_tmp[SEG_NT / WAVE + 1];
+	const int t = threadIdx.x;
+	const int nsub = 1 << sub_bits;           // 4..SEG_MAX_SUB (host-checked)
+	const int per = RS_BINS * nsub / SEG_NT;  // a thread's segments, all of one digit
+	const int gpt = nsub / per;               // threads per digit
+	const int c = t / gpt, q = t % gpt;
+	uint32_t cnt[PER_MAX], sum = 0;
+#pragma unroll
+	for (int k = 0; k < PER_MAX; ++k) {
+		cnt[k] = 0;
+		if (k < per) {
+			const uint32_t raw = cursor[t * per + k];
+			cnt[k] = (int64_t)raw < sub_cap ? raw : (uint32_t)sub_cap;
+			sum += cnt[k];
+		}
+	}
+	s_part[t] = sum;
+	__syncthreads();
+	uint32_t off = 0, tot = 0;
+	for (int u = 0; u < gpt; ++u) {
+		const uint32_t v = s_part[c * gpt + u];
+		off += u < q ? v : 0u;
+		tot += v;
+	}
+	uint32_t* const pre = seg + SEG_PRE + (int64_t)c * (nsub + 1);
+#pragma unroll
+	for (int k = 0; k < PER_MAX; ++k)
+		if (k < per) {
+			pre[q * per + k] = off;
+			off += cnt[k];
+		}
+	if (q == 0) {
+		pre[nsub] = tot;
+		s_tot[c] = tot;
+	}
+	__syncthreads();
+	const uint32_t ct = t < RS_BINS ? s_tot[t] : 0u;
+	uint32_t all_c, all_t;
+	const uint32_t cs = block_excl_scan<SEG_NT>(ct, s_tmp, &all_c);
+	const uint32_t ts = block_excl_scan<SEG_NT>((ct + RS_TILE - 1) / RS_TILE, s_tmp, &all_t);
+	if (t < RS_BINS) {
+		seg[SEG_TSTART + t] = ts;
+		seg[SEG_CSTART + t] = cs;
+	}
+	if (t == 0) {
+		seg[SEG_TSTART + RS_BINS] = all_t;
+		seg[SEG_CSTART + RS_BINS] = all_c;
+	}
+}
+
+// Position p (< sp[nsub]) of coarse digit c in the sub-range layout, given
+// the digit's prefix counts in LDS (sp[0..nsub]): the sub-range is the last
+// j with sp[j] <= p (empty ones are passed over). j carries the sub-range of
+// the lane's previous position (positions only grow; -1 before the first,
+// which is found by bisection), and is updated.
+__device__ __forceinline__ int64_t seg_addr(const uint32_t* sp, int nsub, uint32_t c, int sub_bits,
+                                            int64_t sub_cap, uint32_t p, int& j) {
+	if (j < 0) {
+		int lo = 0, hi = nsub - 1;
+		while (lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			if (sp[mid] <= p)
+				lo = mid;
+			else
+				hi = mid - 1;
+		}
+		j = lo;
+	}
+	while (j + 1 < nsub && sp[j + 1] <= p)
+		++j;
+	return (int64_t)((c << sub_bits) | (uint32_t)j) * sub_cap + (p - sp[j]);
+}
+
+// The tile of workgroup i in the coarse-digit tiling (ts[0..256] in LDS):
+// its digit and its first position inside the digit, or false past the end
+__device__ __forceinline__ bool coarse_tile(const uint32_t* ts, int64_t i, int64_t grid, uint32_t* c_out,
+                                            uint32_t* k_out) {
+	const uint32_t ntiles = ts[RS_BINS];
+	const uint32_t tile = (uint32_t)xcd_tile(i, grid);
+	if (tile >= ntiles)
+		return false;
+	int lo = 0, hi = RS_BINS - 1;  // the coarse digit: largest c with ts[c] <= tile
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (ts[mid] <= tile)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	*c_out = (uint32_t)lo;
+	*k_out = tile - ts[lo];
+	return true;
+}
+
+// The sub-range layout copied into one dense array, digits in order: the
+// exact regrouping after a bucket outgrew its region starts from it.
+__global__ __launch_bounds__(256) void k_seg_compact(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                     const uint32_t* __restrict__ seg, int sub_bits,
+                                                     int64_t sub_cap) {
+	__shared__ uint32_t s_ts[RS_BINS + 1];
+	__shared__ uint32_t s_sp[SEG_MAX_SUB + 1];
+	const int nsub = 1 << sub_bits;
+	for (int i = threadIdx.x; i <= RS_BINS; i += 256)
+		s_ts[i] = seg[SEG_TSTART + i];
+	__syncthreads();
+	uint32_t c, k;
+	if (!coarse_tile(s_ts, blockIdx.x, gridDim.x, &c, &k))  // uniform
+		return;
+	for (int i = threadIdx.x; i <= nsub; i += 256)
+		s_sp[i] = seg[SEG_PRE + (int64_t)c * (nsub + 1) + i];
+	__syncthreads();
+	const uint32_t total = s_sp[nsub];
+	const uint32_t p0 = k * RS_TILE;
+	const uint32_t n = total - p0 < (uint32_t)RS_TILE ? total - p0 : (uint32_t)RS_TILE;
+	const int64_t dst = (int64_t)seg[SEG_CSTART + c] + p0;
+	int j = -1;
+	for (uint32_t i = threadIdx.x; i < n; i += 256)
+		out[dst + i] = in[seg_addr(s_sp, nsub, c, sub_bits, sub_cap, p0 + i, j)];
+}
+
 struct RegionArgs {
-	const uint64_t* in;     // sorted by the coarse digit
+	const uint64_t* in;     // sorted by the coarse digit, or the segment layout
 	uint64_t* out;          // the regions
 	const uint32_t* ctl;    // k_coarse_tiles
 	uint32_t* count;        // 2^B cursors, zeroed
 	int64_t stride;         // slots per region
 	int32_t shift;          // lowest bucket bit: the bucket is bits [shift, shift + B)
 	int32_t fine_bits;      // B - 8
+	// sub-range layout (k_seg_tiles): ctl is then seg
+	int64_t sub_cap;
+	int32_t sub_bits;
+	int32_t pad;
 };
 
-template <int NT = RS_NT>
+template <int NT = RS_NT, bool SEGS = false>
 __global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
 	constexpr int IT = RS_TILE / NT;   // keys per thread
 	constexpr int SEG = IT * WAVE;     // keys per wave
@@ -945,38 +1171,71 @@ __global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
 	__shared__ uint32_t s_gbase[RS_BINS];
 	__shared__ uint32_t s_scan[NT / WAVE + 1];
 	__shared__ uint32_t s_ts[RS_BINS + 1];
+	__shared__ uint32_t s_sp[SEGS ? SEG_MAX_SUB + 1 : 1];
 
 	const int t = threadIdx.x;
 	const int lane = lane_id();
 	const int w = t / WAVE;
+	const uint32_t* const ts = a.ctl + (SEGS ? SEG_TSTART : RS_BINS + 1);  // first tile of each digit
 	for (int i = t; i <= RS_BINS; i += NT)
-		s_ts[i] = a.ctl[RS_BINS + 1 + i];
+		s_ts[i] = ts[i];
 	__syncthreads();
-	const uint32_t ntiles = s_ts[RS_BINS];
-	const uint32_t tile = (uint32_t)xcd_tile(blockIdx.x, gridDim.x);
-	if (tile >= ntiles)  // uniform
+	uint32_t c, kt;
+	if (!coarse_tile(s_ts, blockIdx.x, gridDim.x, &c, &kt))  // uniform
 		return;
-	int lo = 0, hi = RS_BINS - 1;  // the coarse bucket: largest c with s_ts[c] <= tile
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (s_ts[mid] <= tile)
-			lo = mid;
-		else
-			hi = mid - 1;
+	int64_t base;  // the tile's first position: in the input, or inside the digit (SEGS)
+	int tile_n;
+	const int nsub = SEGS ? 1 << a.sub_bits : 1;
+	if constexpr (SEGS) {
+		for (int i = t; i <= nsub; i += NT)
+			s_sp[i] = a.ctl[SEG_PRE + (int64_t)c * (nsub + 1) + i];
+		__syncthreads();
+		base = (int64_t)kt * RS_TILE;
+		const int64_t total = s_sp[nsub];
+		tile_n = (int)(total - base < RS_TILE ? total - base : RS_TILE);  // >= 1
+	} else {
+		const int64_t cbeg = a.ctl[c], cend = a.ctl[c + 1];
+		base = cbeg + (int64_t)kt * RS_TILE;
+		tile_n = (int)(cend - base < RS_TILE ? cend - base : RS_TILE);  // >= 1
 	}
-	const uint32_t c = (uint32_t)lo;
-	const int64_t cbeg = a.ctl[c], cend = a.ctl[c + 1];
-	const int64_t base = cbeg + (int64_t)(tile - s_ts[c]) * RS_TILE;
-	const int tile_n = (int)(cend - base < RS_TILE ? cend - base : RS_TILE);  // >= 1
 
 	const int64_t seg = base + (int64_t)w * SEG;
 	const int wrem = tile_n - w * SEG;
 	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? wrem : SEG);
 	uint64_t k[IT];
+	if constexpr (SEGS) {
+		// the wave's positions [seg, seg + SEG) mostly lie in one or two
+		// sub-ranges: the first one and the next two boundaries, found once
+		// for the wave, place every position without further LDS reads;
+		// with more boundaries inside the range each lane walks them
+		int j0 = -1;
+		(void)seg_addr(s_sp, nsub, c, a.sub_bits, a.sub_cap, (uint32_t)seg, j0);
+		const uint32_t b0 = s_sp[j0];
+		const uint32_t b1 = j0 + 1 < nsub ? s_sp[j0 + 1] : 0xFFFFFFFFu;
+		const uint32_t b2 = j0 + 2 < nsub ? s_sp[j0 + 2] : 0xFFFFFFFFu;
+		const uint32_t b3 = j0 + 3 < nsub ? s_sp[j0 + 3] : 0xFFFFFFFFu;
+		const bool few = (int64_t)b3 >= seg + SEG;
+		int j = j0;
 #pragma unroll
-	for (int r = 0; r < IT; ++r) {
-		const bool valid = r * WAVE + lane < w_valid;
-		k[r] = valid ? __builtin_nontemporal_load(&a.in[seg + (int64_t)r * WAVE + lane]) : 0ull;  // read once
+		for (int r = 0; r < IT; ++r) {
+			const bool valid = r * WAVE + lane < w_valid;
+			const uint32_t p = (uint32_t)(seg + (int64_t)r * WAVE + lane);
+			int64_t at;
+			if (few) {
+				const int jj = j0 + (p >= b1 ? 1 : 0) + (p >= b2 ? 1 : 0);
+				const uint32_t bb = p >= b2 ? b2 : p >= b1 ? b1 : b0;
+				at = (int64_t)((c << a.sub_bits) | (uint32_t)jj) * a.sub_cap + (p - bb);
+			} else {
+				at = valid ? seg_addr(s_sp, nsub, c, a.sub_bits, a.sub_cap, p, j) : 0;
+			}
+			k[r] = valid ? __builtin_nontemporal_load(&a.in[at]) : 0ull;  // read once
+		}
+	} else {
+#pragma unroll
+		for (int r = 0; r < IT; ++r) {
+			const bool valid = r * WAVE + lane < w_valid;
+			k[r] = valid ? __builtin_nontemporal_load(&a.in[seg + (int64_t)r * WAVE + lane]) : 0ull;  // read once
+		}
 	}
 	uint32_t lp[IT];  // slot order inside a bucket is free: the unstable rank
 	rank_tile_u<NT, IT>(k, w_valid, a.shift, lp, s_cnt, s_dpre, s_scan);

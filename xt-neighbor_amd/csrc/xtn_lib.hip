@@ -79,6 +79,7 @@ struct xtn_handle {
 	// workspaces (grow-only)
 	DBuf bcount, ctl;  // region layout: bucket cursors, coarse bucket/tile table
 	DBuf recs_all, recs_part;  // chunked runs: every record once, then ordered by part
+	DBuf seg_cur, seg_ctl;  // the generator's coarse pass: sub-range cursors, digit tiling
 	DBuf pk, voff, recs, recs_alt, hist, scan_tmp, runs_small, runs_big, big_pre, pairs, pairs_alt,
 	    flags, pos, dist, out_pairs, out_dist, ctrs, part_hist, scalar;
 	// overlap-mode workspaces
@@ -104,7 +105,7 @@ struct xtn_handle {
 	struct GraphSig {
 		int64_t nseq, row_base, V, pcap, alloc_gen;
 		const void *seq_map, *set_of, *chars, *offs;
-		int32_t d, measure, bucket_cap, target, sort_nt, sort_tile, select_nt, low;
+		int32_t d, measure, bucket_cap, target, sort_nt, sort_tile, select_nt, low, coarse, pad;
 		bool operator==(const GraphSig& o) const { return memcmp(this, &o, sizeof(*this)) == 0; }
 	};
 	hipGraph_t graph = nullptr;
@@ -112,6 +113,7 @@ struct xtn_handle {
 	GraphSig graph_sig{}, graph_want{};
 	bool graph_want_set = false;
 	int64_t graph_launches = 0;  // kernels the graph holds
+	bool graph_coarse = false;   // the graph's generator does the coarse pass
 	// engine parameters the autotuner chooses (xtn_set_tuning / xtn_autotune)
 	int tune_cap = 3072;      // bucket kernel instantiation: 2048, 3072 or 4096 records
 	int tune_target = 2200;   // mean records per bucket
@@ -120,6 +122,8 @@ struct xtn_handle {
 	int tune_select_nt = 256;   // threads of a select tile (8 candidates each): 256 or 512
 	int tune_low = 0;           // pair bits left unsorted for the select's in-group ranking: 0, 8, 16, 24
 	bool tail_full = false;     // a group outgrew the select's window: full sorts until new sequences
+	bool coarse_off = false;    // a sub-range of the generator's coarse pass overflowed: the separate
+	                            // pass until new sequences
 	double tune_ms = 0, tune_ms_default = 0;
 	bool capturing = false;
 	bool no_alloc = false;  // xtn_nn_noalloc: every buffer is a slice of the static scratch
@@ -779,9 +783,21 @@ struct Grouping {
 	int hi = 64, rs = 0;  // the chunk's hash bits end at hi; runs are equal rec >> rs
 	bool region = false;  // fixed bucket regions (more than 8 bucket bits)
 	bool fused_hist = false;
+	bool coarse = false;  // region layout whose coarse pass the generator does (segment layout in recs_alt)
+	int nseg = 0;
+	int64_t sub_cap = 0;
 	int64_t rtiles = 0;
 	BucketArgs ba{};
 };
+
+// sub-ranges per coarse digit in the generator's coarse pass: a generator
+// tile takes the one of its index mod 64 (tiles of one sub-range run on one
+// XCD), so each slot cursor sees 1/16384 of the tiles' claims (same-address
+// atomics serialise)
+constexpr int COARSE_SUB_BITS = 6;
+static_assert((1 << COARSE_SUB_BITS) <= SEG_MAX_SUB && RS_BINS * (1 << COARSE_SUB_BITS) % SEG_NT == 0 &&
+                  (1 << COARSE_SUB_BITS) % (RS_BINS * (1 << COARSE_SUB_BITS) / SEG_NT) == 0,
+              "k_seg_tiles: whole digits per thread group, at most PER_MAX segments per thread");
 
 // records in recs (written by the generator) -> the bucket layout ba names:
 // region layout (exact pass over the top 8 bucket bits into recs_alt, then
@@ -790,6 +806,34 @@ struct Grouping {
 int issue_grouping(xtn_handle* h, Grouping& g) {
 	hipStream_t s = h->stream;
 	BucketArgs& ba = g.ba;
+	if (g.region && g.coarse) {
+		StageScope sg(h, "sort");
+		k_seg_tiles<<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), COARSE_SUB_BITS, g.sub_cap,
+		                                 h->seg_ctl.as<uint32_t>());
+		HIP_LAUNCH_CHECK(h);
+		k_fill_u64<<<grid_for(g.nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), g.nb / 2, 0ull);
+		HIP_LAUNCH_CHECK(h);
+		RegionArgs ra{};
+		ra.in = h->recs_alt.as<uint64_t>();
+		ra.out = h->recs.as<uint64_t>();
+		ra.ctl = h->seg_ctl.as<uint32_t>();
+		ra.count = h->bcount.as<uint32_t>();
+		ra.stride = ba.cap;
+		ra.shift = g.hi - g.B;
+		ra.fine_bits = g.B - 8;
+		ra.sub_cap = g.sub_cap;
+		ra.sub_bits = COARSE_SUB_BITS;
+		{
+			ProfScope ps(h, "region", 16.0 * (double)g.nrec);
+			k_region_scatter<RS_NT, true><<<(unsigned)(g.rtiles + RS_BINS), RS_NT, 0, s>>>(ra);
+			HIP_LAUNCH_CHECK(h);
+		}
+		ba.rec = h->recs.as<uint64_t>();
+		ba.bcount = h->bcount.as<uint32_t>();
+		ba.bstride = ba.cap;
+		ba.bstart = nullptr;
+		return XTN_OK;
+	}
 	if (g.region) {
 		StageScope sg(h, "sort");
 		uint32_t* hist = h->hist.as<uint32_t>();
@@ -847,7 +891,15 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 // bucket bits in recs_alt by the first pass, are sorted exactly by every
 // bucket bit and bucketed by bounds
 int regroup_exact(xtn_handle* h, Grouping& g) {
-	std::swap(h->recs, h->recs_alt);
+	if (g.coarse) {
+		// the generator's segment layout, compacted into recs
+		k_seg_compact<<<(unsigned)(g.rtiles + RS_BINS), 256, 0, h->stream>>>(
+		    h->recs_alt.as<uint64_t>(), h->recs.as<uint64_t>(), h->seg_ctl.as<uint32_t>(), COARSE_SUB_BITS, g.sub_cap);
+		HIP_LAUNCH_CHECK(h);
+		g.coarse = false;
+	} else {
+		std::swap(h->recs, h->recs_alt);
+	}
 	g.region = false;
 	g.fused_hist = false;
 	return issue_grouping(h, g);
@@ -1167,8 +1219,22 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)cap_recs));
 		if (g.region)
 			TRY(ensure(h, h->recs, sizeof(uint64_t) * (size_t)((int64_t)bucket_cap << g.B)));
+		// a single chunk on the region layout: the generator does the coarse
+		// pass itself, into 2^COARSE_SUB_BITS sub-ranges per coarse digit of
+		// sub_cap slots (the mean with 10 % and eight standard deviations of
+		// headroom; a sub-range that still overflows sends the run back to the
+		// separate pass)
+		g.coarse = g.region && nparts == 1 && !h->coarse_off && !h->tail_safe && h->cfg.coarse_separate == 0;
+		if (g.coarse) {
+			g.nseg = RS_BINS << COARSE_SUB_BITS;
+			const double mean = (double)nrec / g.nseg;
+			g.sub_cap = (int64_t)(mean * 1.1 + 8.0 * std::sqrt(mean) + 64.0);
+			TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)(g.sub_cap * g.nseg)));
+			TRY(ensure(h, h->seg_cur, sizeof(uint32_t) * (size_t)g.nseg));
+			TRY(ensure(h, h->seg_ctl, sizeof(uint32_t) * (size_t)seg_words(1 << COARSE_SUB_BITS)));
+		}
 		// the single-chunk generator also counts the first sort digit per tile
-		g.fused_hist = nparts == 1 && g.B > 0;
+		g.fused_hist = nparts == 1 && g.B > 0 && !g.coarse;
 		g.rtiles = (nrec + RS_TILE - 1) / RS_TILE;
 		TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * g.rtiles)));
 		BucketArgs& ba = g.ba;
@@ -1184,8 +1250,11 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		ba.n_ovf = ctr_field(h, offsetof(RunCtrs, append));
 		return XTN_OK;
 	};
-	// the generator launch of one chunk (records into recs)
+	// the generator launch of one chunk (records into recs, or the segment
+	// layout in recs_alt when it does the coarse pass)
+	bool used_coarse = false;
 	auto issue_gen = [&](int part, const Grouping& g) -> int {
+		used_coarse = false;
 		ga.out = h->recs.as<uint64_t>();
 		ga.cap = cap_recs;
 		if (g.fused_hist) {
@@ -1196,7 +1265,23 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ga.digit_hist = nullptr;
 		}
 		StageScope sg(h, "generate");
-		if (nparts == 1) {
+		if (nparts == 1 && g.coarse) {
+			static_assert((RS_BINS << COARSE_SUB_BITS) % 2 == 0, "cursors cleared in u64 words");
+			k_fill_u64<<<grid_for(g.nseg / 2, 256), 256, 0, s>>>(h->seg_cur.as<unsigned long long>(), g.nseg / 2, 0ull);
+			HIP_LAUNCH_CHECK(h);
+			ga.out = h->recs_alt.as<uint64_t>();
+			ga.cap = g.sub_cap * g.nseg;
+			ga.digit_hist = nullptr;
+			ga.digit_shift = hi - 8;
+			ga.sub_bits = COARSE_SUB_BITS;
+			ga.cursor = h->seg_cur.as<uint32_t>();
+			ga.coarse_ovf = ctr_field(h, offsetof(RunCtrs, coarse_ovf));
+			ga.sub_cap = g.sub_cap;
+			ProfScope ps(h, "gen_coarse", (double)V * 8.0 + (double)nseq * 24.0);
+			k_gen<GEN_COARSE, GEN_COARSE_NT><<<gen_grid, GEN_COARSE_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+			used_coarse = true;
+		} else if (nparts == 1) {
 			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (g.fused_hist ? 1024.0 * g.rtiles : 0.0));
 			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
 			HIP_LAUNCH_CHECK(h);
@@ -1251,6 +1336,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			// everything issued, one readback: the whole run of a single
 			// round is a graph once it has run the same way before
 			auto issue_all = [&]() -> int {
+				// counters first: the generator's coarse pass may flag an overflow
+				TRY(zero_ctl(h, true, true));
 				if (!grouped) {
 					TRY(prologue());
 					TRY(plan_chunk((int64_t)V, g1));
@@ -1259,7 +1346,6 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 					grouped = true;
 				}
 				TRY(tail_prepare(h, pcap, idb, out_total, &tp));
-				TRY(zero_ctl(h, true, true));
 				TRY(issue_buckets(h, g1, lo, hi_id, pcap));
 				StageScope sg(h, "verify");
 				TRY(issue_tail(h, tp, idb, seq_map, row_base, set_of, out_total));
@@ -1286,11 +1372,14 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				sig.sort_tile = h->tune_sort_tile;
 				sig.select_nt = h->tune_select_nt;
 				sig.low = h->tail_full ? 0 : h->tune_low;
+				sig.coarse = h->coarse_off || h->cfg.coarse_separate ? 0 : 1;
+				sig.pad = 0;
 			}
 			bool via_graph = false;
 			if (graph_ok && h->graph_exec && sig == h->graph_sig) {
 				HIP_TRY(h, hipGraphLaunch(h->graph_exec, s));
 				st.n_launches += h->graph_launches;
+				used_coarse = h->graph_coarse;
 				via_graph = true;
 			} else if (graph_ok && h->graph_want_set && sig == h->graph_want) {
 				// the previous run had this signature: capture this one (every
@@ -1315,6 +1404,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 					h->graph_exec = exec;
 					h->graph_sig = sig;
 					h->graph_launches = st.n_launches - l0;
+					h->graph_coarse = used_coarse;
 					HIP_TRY(h, hipGraphLaunch(exec, s));
 					via_graph = true;
 				} else {
@@ -1332,6 +1422,15 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				TRY(issue_all());
 			}
 			TRY(read_head(h, &head));
+			if (head.rc.coarse_ovf) {
+				// a sub-range of the generator's coarse pass overflowed and lost
+				// records: the run again with the separate pass, which these
+				// sequences keep from now on
+				h->coarse_off = true;
+				grouped = false;
+				prologue_done = false;
+				continue;
+			}
 			const bool clean =
 			    !head.error && !head.overflow && (int64_t)head.rc.n_cand <= pcap && head.rc.append == 0;
 			if (head.overflow)
@@ -1419,6 +1518,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		++st.n_rounds;
 		st.tail_checked = h->tail_safe ? 1 : 0;
 		st.tail_passes = h->tail_safe ? (2 * idb + 7) / 8 : tail_passes(h, idb);
+		st.coarse_in_generator = used_coarse ? 1 : 0;
 		rounds.pop_front();
 	}
 	h->last_pairs = max_round;
@@ -1675,7 +1775,7 @@ void xtn_close(xtn_handle* h) {
 		return;
 	(void)hipSetDevice(h->cfg.device);
 	(void)hipStreamSynchronize(h->stream);
-	DBuf* bufs[] = {&h->recs_all, &h->recs_part, &h->bcount, &h->ctl, &h->chars, &h->offs, &h->pk, &h->voff,
+	DBuf* bufs[] = {&h->recs_all, &h->recs_part, &h->seg_cur, &h->seg_ctl, &h->bcount, &h->ctl, &h->chars, &h->offs, &h->pk, &h->voff,
 	                &h->recs, &h->recs_alt, &h->hist,
 	                &h->scan_tmp, &h->runs_small, &h->runs_big, &h->big_pre, &h->pairs,
 	                &h->pairs_alt, &h->flags, &h->pos, &h->dist, &h->out_pairs, &h->out_dist,
@@ -1873,6 +1973,7 @@ int xtn_set_sequences(xtn_handle* h, const char* chars, const uint32_t* offsets,
 	h->vprefix.clear();
 	h->vprefix_d = 0;
 	h->tail_full = false;
+	h->coarse_off = false;
 	return XTN_OK;
 }
 

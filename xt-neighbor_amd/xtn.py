@@ -70,7 +70,7 @@ class Config(ctypes.Structure):
                 ("max_records_per_chunk", ctypes.c_int64),
                 ("output_chunk_pairs", ctypes.c_int64),
                 ("profile", ctypes.c_int32), ("bucket_capacity", ctypes.c_int32),
-                ("no_graph", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("no_graph", ctypes.c_int32), ("coarse_separate", ctypes.c_int32),
                 ("max_pairs_per_round", ctypes.c_int64)]
 
 
@@ -91,7 +91,7 @@ class Stats(ctypes.Structure):
                 ("ms_output", ctypes.c_double), ("n_rounds", ctypes.c_int64),
                 ("tail_checked", ctypes.c_int64), ("n_launches", ctypes.c_int64),
                 ("n_syncs", ctypes.c_int64), ("graph_replays", ctypes.c_int64),
-                ("tail_passes", ctypes.c_int64)]
+                ("tail_passes", ctypes.c_int64), ("coarse_in_generator", ctypes.c_int64)]
 
 
 class Tuning(ctypes.Structure):
