@@ -784,6 +784,8 @@ struct Grouping {
 	bool region = false;  // fixed bucket regions (more than 8 bucket bits)
 	bool fused_hist = false;
 	bool coarse = false;  // region layout whose coarse pass the generator does (segment layout in recs_alt)
+	const uint64_t* src = nullptr;  // region layout: the chunk's records read from here, not from recs
+	                                // (its slice of the part-ordered records, left intact for later rounds)
 	int nseg = 0;
 	int64_t sub_cap = 0;
 	int64_t rtiles = 0;
@@ -837,9 +839,10 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 	if (g.region) {
 		StageScope sg(h, "sort");
 		uint32_t* hist = h->hist.as<uint32_t>();
+		const uint64_t* in = g.src ? g.src : h->recs.as<uint64_t>();
 		if (!g.fused_hist) {
 			ProfScope ps(h, "sort_rec_hist", (double)g.nrec * 8.0);
-			k_radix_hist<<<(unsigned)g.rtiles, RS_NT, 0, s>>>(h->recs.as<uint64_t>(), g.nrec, g.hi - 8, hist, g.rtiles);
+			k_radix_hist<<<(unsigned)g.rtiles, RS_NT, 0, s>>>(in, g.nrec, g.hi - 8, hist, g.rtiles);
 			HIP_LAUNCH_CHECK(h);
 		}
 		uint32_t* dtot = nullptr;
@@ -849,7 +852,7 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 		{
 			ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)g.nrec);
 			k_radix_scatter<false, RS_NT, true><<<(unsigned)g.rtiles, RS_NT, 0, s>>>(
-			    h->recs.as<uint64_t>(), nullptr, h->recs_alt.as<uint64_t>(), nullptr, g.nrec, g.hi - 8, hist, g.rtiles);
+			    in, nullptr, h->recs_alt.as<uint64_t>(), nullptr, g.nrec, g.hi - 8, hist, g.rtiles);
 			HIP_LAUNCH_CHECK(h);
 		}
 		k_fill_u64<<<grid_for(g.nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), g.nb / 2, 0ull);
@@ -900,6 +903,7 @@ int regroup_exact(xtn_handle* h, Grouping& g) {
 	} else {
 		std::swap(h->recs, h->recs_alt);
 	}
+	g.src = nullptr;
 	g.region = false;
 	g.fused_hist = false;
 	return issue_grouping(h, g);
@@ -1253,7 +1257,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	// the generator launch of one chunk (records into recs, or the segment
 	// layout in recs_alt when it does the coarse pass)
 	bool used_coarse = false;
-	auto issue_gen = [&](int part, const Grouping& g) -> int {
+	auto issue_gen = [&](int part, Grouping& g) -> int {
 		used_coarse = false;
 		ga.out = h->recs.as<uint64_t>();
 		ga.cap = cap_recs;
@@ -1285,6 +1289,9 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (g.fused_hist ? 1024.0 * g.rtiles : 0.0));
 			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
 			HIP_LAUNCH_CHECK(h);
+		} else if (gen_once && g.region) {
+			// the region layout's first pass reads the part's slice in place
+			g.src = h->recs_part.as<uint64_t>() + part_start[part];
 		} else if (gen_once) {
 			ProfScope ps(h, "part_copy", 16.0 * (double)part_count[part]);
 			if (part_count[part])
