@@ -1014,52 +1014,73 @@ __global__ __launch_bounds__(RS_BINS) void k_coarse_tiles(const uint32_t* __rest
 // would be. k_seg_tiles writes, in `seg`: the first tile of every coarse
 // digit (SEG_TSTART, 257 words, as k_coarse_tiles does), the digit's first
 // record in a dense layout (SEG_CSTART, 257 words) and the prefix counts of
-// its sub-ranges (SEG_PRE, S + 1 words per digit).
+// its sub-ranges (SEG_PRE, S + 1 words per digit at a stride of S + 4, so
+// that every digit's row starts on 16 bytes). It also clears the region
+// layout's bucket cursors, one launch fewer.
 constexpr int SEG_NT = 1024;
 constexpr int SEG_MAX_SUB = 64;  // sub-ranges per coarse digit, at most
 constexpr int SEG_TSTART = 0;
 constexpr int SEG_CSTART = RS_BINS + 1;
-constexpr int SEG_PRE = 2 * (RS_BINS + 1);
-constexpr int seg_words(int nsub) { return SEG_PRE + RS_BINS * (nsub + 1); }
+constexpr int SEG_PRE = 2 * (RS_BINS + 1) + 2;  // 16-byte aligned
+constexpr int seg_stride(int nsub) { return nsub + 4; }
+constexpr int seg_words(int nsub) { return SEG_PRE + RS_BINS * seg_stride(nsub); }
 
-__global__ __launch_bounds__(SEG_NT) void k_seg_tiles(const uint32_t* __restrict__ cursor, int sub_bits,
-                                                      int64_t sub_cap, uint32_t* __restrict__ seg) {
-	constexpr int PER_MAX = RS_BINS * SEG_MAX_SUB / SEG_NT;  // segments per thread, at most
+template <int NSUB>
+__global__ __launch_bounds__(SEG_NT) void k_seg_tiles(const uint32_t* __restrict__ cursor, int64_t sub_cap,
+                                                      uint32_t* __restrict__ seg, uint32_t* __restrict__ bzero,
+                                                      int64_t nzero) {
+	constexpr int PER = RS_BINS * NSUB / SEG_NT;  // a thread's segments, all of one digit
+	constexpr int GPT = NSUB / PER;               // threads per digit
+	static_assert(PER % 4 == 0 && NSUB % PER == 0 && NSUB <= SEG_MAX_SUB, "whole digits, u32x4 rows");
 	__shared__ uint32_t s_part[SEG_NT];
 	__shared__ uint32_t s_tot[RS_BINS];
 	__shared__ uint32_t s_tmp[SEG_NT / WAVE + 1];
 	const int t = threadIdx.x;
-	const int nsub = 1 << sub_bits;           // 4..SEG_MAX_SUB (host-checked)
-	const int per = RS_BINS * nsub / SEG_NT;  // a thread's segments, all of one digit
-	const int gpt = nsub / per;               // threads per digit
-	const int c = t / gpt, q = t % gpt;
-	uint32_t cnt[PER_MAX], sum = 0;
+	const int c = t / GPT, q = t % GPT;
+	uint32_t cnt[PER], sum = 0;
+	const uint4* const cur4 = reinterpret_cast<const uint4*>(cursor + (int64_t)t * PER);
 #pragma unroll
-	for (int k = 0; k < PER_MAX; ++k) {
-		cnt[k] = 0;
-		if (k < per) {
-			const uint32_t raw = cursor[t * per + k];
-			cnt[k] = (int64_t)raw < sub_cap ? raw : (uint32_t)sub_cap;
-			sum += cnt[k];
-		}
+	for (int k = 0; k < PER / 4; ++k) {
+		const uint4 v = cur4[k];
+		cnt[4 * k] = v.x;
+		cnt[4 * k + 1] = v.y;
+		cnt[4 * k + 2] = v.z;
+		cnt[4 * k + 3] = v.w;
 	}
+#pragma unroll
+	for (int k = 0; k < PER; ++k) {
+		cnt[k] = (int64_t)cnt[k] < sub_cap ? cnt[k] : (uint32_t)sub_cap;
+		sum += cnt[k];
+	}
+	uint4* const z4 = reinterpret_cast<uint4*>(bzero);
+	for (int64_t i = t; i < nzero / 4; i += SEG_NT)
+		z4[i] = make_uint4(0u, 0u, 0u, 0u);
 	s_part[t] = sum;
 	__syncthreads();
 	uint32_t off = 0, tot = 0;
-	for (int u = 0; u < gpt; ++u) {
-		const uint32_t v = s_part[c * gpt + u];
+#pragma unroll
+	for (int u = 0; u < GPT; ++u) {
+		const uint32_t v = s_part[c * GPT + u];
 		off += u < q ? v : 0u;
 		tot += v;
 	}
-	uint32_t* const pre = seg + SEG_PRE + (int64_t)c * (nsub + 1);
+	uint32_t* const pre = seg + SEG_PRE + (int64_t)c * seg_stride(NSUB);
+	uint4* const pre4 = reinterpret_cast<uint4*>(pre + q * PER);
 #pragma unroll
-	for (int k = 0; k < PER_MAX; ++k)
-		if (k < per) {
-			pre[q * per + k] = off;
-			off += cnt[k];
-		}
+	for (int k = 0; k < PER / 4; ++k) {
+		uint4 v;
+		v.x = off;
+		off += cnt[4 * k];
+		v.y = off;
+		off += cnt[4 * k + 1];
+		v.z = off;
+		off += cnt[4 * k + 2];
+		v.w = off;
+		off += cnt[4 * k + 3];
+		pre4[k] = v;
+	}
 	if (q == 0) {
-		pre[nsub] = tot;
+		pre[NSUB] = tot;
 		s_tot[c] = tot;
 	}
 	__syncthreads();
@@ -1136,7 +1157,7 @@ __global__ __launch_bounds__(256) void k_seg_compact(const uint64_t* __restrict_
 	if (!coarse_tile(s_ts, blockIdx.x, gridDim.x, &c, &k))  // uniform
 		return;
 	for (int i = threadIdx.x; i <= nsub; i += 256)
-		s_sp[i] = seg[SEG_PRE + (int64_t)c * (nsub + 1) + i];
+		s_sp[i] = seg[SEG_PRE + (int64_t)c * seg_stride(nsub) + i];
 	__syncthreads();
 	const uint32_t total = s_sp[nsub];
 	const uint32_t p0 = k * RS_TILE;
@@ -1188,7 +1209,7 @@ __global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
 	const int nsub = SEGS ? 1 << a.sub_bits : 1;
 	if constexpr (SEGS) {
 		for (int i = t; i <= nsub; i += NT)
-			s_sp[i] = a.ctl[SEG_PRE + (int64_t)c * (nsub + 1) + i];
+			s_sp[i] = a.ctl[SEG_PRE + (int64_t)c * seg_stride(nsub) + i];
 		__syncthreads();
 		base = (int64_t)kt * RS_TILE;
 		const int64_t total = s_sp[nsub];

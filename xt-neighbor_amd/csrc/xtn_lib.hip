@@ -797,9 +797,6 @@ struct Grouping {
 // XCD), so each slot cursor sees 1/16384 of the tiles' claims (same-address
 // atomics serialise)
 constexpr int COARSE_SUB_BITS = 6;
-static_assert((1 << COARSE_SUB_BITS) <= SEG_MAX_SUB && RS_BINS * (1 << COARSE_SUB_BITS) % SEG_NT == 0 &&
-                  (1 << COARSE_SUB_BITS) % (RS_BINS * (1 << COARSE_SUB_BITS) / SEG_NT) == 0,
-              "k_seg_tiles: whole digits per thread group, at most PER_MAX segments per thread");
 
 // records in recs (written by the generator) -> the bucket layout ba names:
 // region layout (exact pass over the top 8 bucket bits into recs_alt, then
@@ -810,10 +807,8 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 	BucketArgs& ba = g.ba;
 	if (g.region && g.coarse) {
 		StageScope sg(h, "sort");
-		k_seg_tiles<<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), COARSE_SUB_BITS, g.sub_cap,
-		                                 h->seg_ctl.as<uint32_t>());
-		HIP_LAUNCH_CHECK(h);
-		k_fill_u64<<<grid_for(g.nb / 2, 256), 256, 0, s>>>(h->bcount.as<unsigned long long>(), g.nb / 2, 0ull);
+		k_seg_tiles<1 << COARSE_SUB_BITS><<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), g.sub_cap,
+		                                                       h->seg_ctl.as<uint32_t>(), h->bcount.as<uint32_t>(), g.nb);
 		HIP_LAUNCH_CHECK(h);
 		RegionArgs ra{};
 		ra.in = h->recs_alt.as<uint64_t>();
