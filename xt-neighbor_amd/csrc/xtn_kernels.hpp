@@ -70,15 +70,11 @@ __device__ __forceinline__ uint64_t variant_count(int L, int d) {
 // form, set_of non-null; those bits lie above every residue slot, so they
 // ride through the deletions into the variant hash) and bits 56..63 the
 // prefix length.
-__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
-                                              const uint32_t* __restrict__ offs,
-                                              const uint32_t* __restrict__ seq_map, uint32_t row_base,
-                                              const uint32_t* __restrict__ set_of,
-                                              int32_t n, int distance, uint64_t* __restrict__ pk,
-                                              uint64_t* __restrict__ vcount) {
-	const int32_t i = blockIdx.x * 256 + threadIdx.x;
-	if (i >= n)
-		return;
+// Returns sequence i's variant count.
+__device__ __forceinline__ uint32_t pack_one(const uint8_t* __restrict__ chars, const uint32_t* __restrict__ offs,
+                                             const uint32_t* __restrict__ seq_map, uint32_t row_base,
+                                             const uint32_t* __restrict__ set_of, int32_t i, int distance,
+                                             uint64_t* __restrict__ pk) {
 	const uint32_t o = seq_map ? seq_map[i] : row_base + (uint32_t)i;
 	const uint32_t s = offs[o];
 	const uint32_t len = offs[o + 1] - s;
@@ -116,7 +112,19 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
 		hi |= (uint64_t)(set_of[i] & PK_SET_MASK) << PK_SET_SHIFT;
 	pk[2 * (int64_t)i] = lo;
 	pk[2 * (int64_t)i + 1] = hi | ((uint64_t)L << 56);
-	vcount[i] = variant_count(L, distance);
+	return (uint32_t)variant_count(L, distance);
+}
+
+__global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ chars,
+                                              const uint32_t* __restrict__ offs,
+                                              const uint32_t* __restrict__ seq_map, uint32_t row_base,
+                                              const uint32_t* __restrict__ set_of,
+                                              int32_t n, int distance, uint64_t* __restrict__ pk,
+                                              uint64_t* __restrict__ vcount) {
+	const int32_t i = blockIdx.x * 256 + threadIdx.x;
+	if (i >= n)
+		return;
+	vcount[i] = pack_one(chars, offs, seq_map, row_base, set_of, i, distance, pk);
 }
 
 // ---------------------------------------------------------- generation --

@@ -93,7 +93,7 @@ struct xtn_handle {
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAcc> prof;
 	std::vector<PendingEvent> spans;  // stage timing, always on
-	DBuf bstart, ovf, set_of, set_offs, tseq, tail_state;
+	DBuf bstart, ovf, set_of, set_offs, tseq, tail_state, pack_state;
 	int64_t last_pairs = 0;
 	int64_t last_unique = 0, last_out = 0;  // the previous run's unique candidates and outputs
 	                                         // (byte counts of the output select's profile)
@@ -1100,9 +1100,28 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	auto prologue = [&]() -> int {
 		if (prologue_done)
 			return XTN_OK;
-		if (v_known)
-			TRY(issue_pack());
-		TRY(tile_seq_table(h, s, nseq, V, &ga.tile_seq));
+		if (v_known && nparts == 1 && !h->tail_safe) {
+			// packing, offsets and tile table in one launch (its look-back
+			// shares the error flag the single-chunk run reads at its end)
+			const int64_t nb = (nseq + PK_NT - 1) / PK_NT;
+			const uint64_t ntiles = (V + GEN_TILE - 1) / GEN_TILE;
+			TRY(ensure(h, h->pack_state, sizeof(uint64_t) * (size_t)nb));
+			TRY(ensure(h, h->tseq, sizeof(uint32_t) * (size_t)(ntiles + 1)));
+			StageScope sg(h, "generate");
+			k_fill_u64<<<grid_for(nb, 256), 256, 0, s>>>(h->pack_state.as<unsigned long long>(), nb, 0ull);
+			HIP_LAUNCH_CHECK(h);
+			ProfScope ps(h, "pack", (double)nseq * (16.0 + 16.0 + 8.0));
+			k_pack_scan<<<(unsigned)nb, PK_NT, 0, s>>>(h->chars.as<uint8_t>(), h->offs.as<uint32_t>(), seq_map, row_base,
+			                                         set_of, nseq, d, h->pk.as<uint64_t>(), h->voff.as<uint64_t>(),
+			                                         h->tseq.as<uint32_t>(), h->pack_state.as<uint64_t>(),
+			                                         LB_SPIN_LIMIT, &ctl_block(h)->tc.error);
+			HIP_LAUNCH_CHECK(h);
+			ga.tile_seq = h->tseq.as<uint32_t>();
+		} else {
+			if (v_known)
+				TRY(issue_pack());
+			TRY(tile_seq_table(h, s, nseq, V, &ga.tile_seq));
+		}
 		prologue_done = true;
 		return XTN_OK;
 	};
@@ -1451,7 +1470,10 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			if (head.error) {
 				// a look-back wait hit its limit: this handle keeps to the
 				// host-checked tail from now on, and the round is redone
+				// (packing included, whose offsets share the flag)
 				h->tail_safe = true;
+				grouped = false;
+				prologue_done = false;
 				continue;
 			}
 			if (head.overflow)
@@ -1784,7 +1806,7 @@ void xtn_close(xtn_handle* h) {
 	                &h->ctrs, &h->part_hist, &h->scalar, &h->ov_keys, &h->ov_keys_alt, &h->ov_vals,
 	                &h->ov_vals_alt, &h->ov_info, &h->ov_leader, &h->ov_ustart, &h->ov_umap,
 	                &h->ov_cnt, &h->ov_matrix, &h->ov_present, &h->bstart, &h->ovf, &h->set_of,
-	                &h->set_offs, &h->tseq, &h->tail_state};
+	                &h->set_offs, &h->tseq, &h->tail_state, &h->pack_state};
 	for (DBuf* b : bufs)
 		release(*b);
 	for (auto& pe : h->pending) {
@@ -2577,6 +2599,7 @@ bool carve_scratch(xtn_handle* h, uint8_t* base) {
 	const int64_t sort_tiles = NA_MAX_PAIRS / RS_TILE + 1, sel_tiles = NA_MAX_PAIRS / 2048 + 1;
 	take(h->pk, 16 * NA_MAX_SEQS);
 	take(h->voff, 8 * (NA_MAX_SEQS + 1));
+	take(h->pack_state, 8 * (NA_MAX_SEQS / 256 + 1));
 	take(h->tseq, 4 * (NA_MAX_RECORDS / GEN_TILE + 2));
 	take(h->recs, 8 * NA_MAX_RECORDS);
 	take(h->recs_alt, 8 * NA_MAX_RECORDS);

@@ -756,4 +756,57 @@ inline SelectFn select_kernel(int distance, int measure, int nt, bool grouped) {
 	return nt == 512 ? select_kernel_nt<512, false>(distance, measure) : select_kernel_nt<256, false>(distance, measure);
 }
 
+// ------------------------------------------- packing with its offsets --
+
+// k_pack, the exclusive scan of the variant counts into voff and the
+// generator's tile table in one launch: workgroup b packs sequences
+// [PK_NT b, PK_NT b + PK_NT), scans their counts, and adds the counts of the
+// workgroups before it by a wave-wide decoupled look-back over 64-bit status
+// words (flag | 62-bit sum; state[] cleared before the launch). Each sequence
+// then writes its offset and records the generator tiles whose first record
+// it holds (tile_seq, as k_tile_seq); the last one writes voff[n] = V and
+// tile_seq[ceil(V / GEN_TILE)], since it holds record V - 1 (every sequence
+// has at least one variant, itself). The waits are on lower-numbered
+// workgroups, which each XCD dispatches first; a wait past spin_limit
+// raises *error and the host redoes the run.
+constexpr int PK_NT = 1024;  // sequences per workgroup: a short look-back chain
+
+__global__ __launch_bounds__(PK_NT) void k_pack_scan(const uint8_t* __restrict__ chars,
+                                                   const uint32_t* __restrict__ offs,
+                                                   const uint32_t* __restrict__ seq_map, uint32_t row_base,
+                                                   const uint32_t* __restrict__ set_of, int32_t n, int distance,
+                                                   uint64_t* __restrict__ pk, uint64_t* __restrict__ voff,
+                                                   uint32_t* __restrict__ tile_seq, uint64_t* __restrict__ state,
+                                                   uint32_t spin_limit, uint32_t* __restrict__ error) {
+	__shared__ uint32_t s_tmp[PK_NT / WAVE + 1];
+	__shared__ uint64_t s_prefix;
+	const int t = threadIdx.x;
+	const int32_t i = blockIdx.x * PK_NT + t;
+	const uint32_t c = i < n ? pack_one(chars, offs, seq_map, row_base, set_of, i, distance, pk) : 0u;
+	uint32_t tot;
+	const uint32_t ex = block_excl_scan<PK_NT>(c, s_tmp, &tot);  // <= PK_NT * 2^18
+	const int64_t b = blockIdx.x;
+	if (t < WAVE) {
+		if (t == 0)
+			lb_store64(state + b, (b == 0 ? LB2_INC : LB2_AGG) | (uint64_t)tot);
+		const uint64_t prefix = b == 0 ? 0ull : lookback_wave(state, b, spin_limit, error);
+		if (t == 0) {
+			if (b > 0)
+				lb_store64(state + b, LB2_INC | (prefix + tot));
+			s_prefix = prefix;
+		}
+	}
+	__syncthreads();
+	if (i >= n)
+		return;
+	const uint64_t o = s_prefix + ex, e = o + c;
+	voff[i] = o;
+	for (uint64_t bt = (o + GEN_TILE - 1) / GEN_TILE; bt * GEN_TILE < e; ++bt)
+		tile_seq[bt] = (uint32_t)i;
+	if (i == n - 1) {
+		voff[n] = e;
+		tile_seq[(e + GEN_TILE - 1) / GEN_TILE] = (uint32_t)i;
+	}
+}
+
 }  // namespace xtn
