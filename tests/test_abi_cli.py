@@ -1,5 +1,6 @@
 """CPU-side checks of the boundary: libxtn.so loads and exports every
-symbol include/xtn.h declares; the xt_neighbor CLI's argument and input
+symbol include/xtn.h declares, and the ctypes mirrors of its structs have
+the header's layout; the xt_neighbor CLI's argument and input
 parsing paths (which finish before the GPU is touched) reproduce the
 reference's messages and exit codes (cli.cu:29-193, airr.cu:116-151)."""
 import ctypes
@@ -38,6 +39,31 @@ def test_version_status_and_defaults():
     cfg = xtn.Config()
     lib.xtn_config_init(ctypes.byref(cfg))
     assert (cfg.distance, cfg.measure, cfg.device, cfg.max_records_per_chunk) == (1, 0, 0, 0)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # every struct the bindings mirror has the header's size and field offsets
+    # (a C program built against include/xtn.h prints them)
+    structs = {"xtn_int2": xtn.Int2, "xtn_seq_info": xtn.SeqInfo, "xtn_output": xtn.Output,
+               "xtn_config": xtn.Config, "xtn_server_config": xtn.ServerConfig, "xtn_stats": xtn.Stats,
+               "xtn_tuning": xtn.Tuning}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "xtn.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), "-o", str(exe), str(src)],
+                   check=True, timeout=60)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True, timeout=60).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
 
 
 def test_open_rejects_bad_config():
