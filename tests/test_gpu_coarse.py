@@ -76,3 +76,25 @@ def test_region_overflow_after_generator_pass(cap):
         got, st = run(eng, distance=2, measure=0, bucket_capacity=cap, coarse_separate=0)
         assert np.array_equal(got, want)
         assert st["coarse_in_generator"] == 1
+
+
+@pytest.mark.parametrize("seed,n,d,measure,dup", [(71, 60_000, 1, 0, 0.0), (72, 150_000, 1, 1, 0.2),
+                                                  (73, 80_000, 2, 0, 0.3), (74, 50_000, 2, 1, 0.0),
+                                                  (75, 25_000, 3, 1, 0.1)])
+def test_generator_pass_sweep(seed, n, d, measure, dup):
+    # short and long strings (lengths 4..30), planted relatives, and a share
+    # of rows repeated a few times each, shuffled: uneven sub-range fills
+    chars, offsets = xtn.synth_nn(seed, n, 14.0, 4.0, 4, 30, planted=0.3, max_edits=d)
+    seqs = xtn.decode(chars, offsets)
+    rng = np.random.default_rng(seed)
+    extra = [seqs[i] for i in rng.integers(0, len(seqs), int(dup * len(seqs)))]
+    seqs = seqs + extra
+    seqs = [seqs[i] for i in rng.permutation(len(seqs))]
+    chars, offsets = xtn.encode(seqs)
+    want = oracle.nn_port_mt(chars, offsets, d, measure, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=d) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        for _ in range(2):
+            got, st = run(eng, distance=d, measure=measure, coarse_separate=0)
+            assert np.array_equal(got, want)
+            assert st["coarse_in_generator"] == 1
