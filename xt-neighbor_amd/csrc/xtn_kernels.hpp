@@ -421,6 +421,12 @@ struct GenArgs {
 	int64_t sub_cap;                // GEN_COARSE: slots per sub-range
 };
 
+// First slot of sub-range j of coarse digit c in the generator's coarse
+// pass (digit-major: a digit's sub-ranges are adjacent)
+__device__ __forceinline__ int64_t seg_base(uint32_t c, uint32_t j, int sub_bits, int64_t sub_cap) {
+	return (int64_t)((c << sub_bits) | j) * sub_cap;
+}
+
 __device__ __forceinline__ int64_t upper_bound_u64(const uint64_t* a, int64_t n, uint64_t x) {
 	int64_t lo = 0, hi = n;
 	while (lo < hi) {
@@ -804,7 +810,7 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 				const uint32_t dg = (uint32_t)(key >> a.digit_shift) & 0xFFu;
 				const int64_t slot = (int64_t)s_gbase[dg] + ((int)lo_pos + i - (int)s_dpre[dg]);
 				if (slot < a.sub_cap)
-					a.out[(int64_t)((dg << a.sub_bits) | sub) * a.sub_cap + slot] = key;
+					a.out[seg_base(dg, sub, a.sub_bits, a.sub_cap) + slot] = key;
 			}
 			__syncthreads();  // the half is written before the next one is placed
 		}
@@ -1014,9 +1020,9 @@ __global__ __launch_bounds__(RS_BINS) void k_coarse_tiles(const uint32_t* __rest
 }
 
 // The coarse pass fused into the generator (k_gen<GEN_COARSE>) leaves the
-// records in sub-ranges: sub-range j of coarse digit c, segment
-// e = (c << sub_bits) | j, holds min(cursor[e], sub_cap) records from
-// e * sub_cap. The passes after it read a coarse digit's sub-ranges as one
+// records in sub-ranges: sub-range j of coarse digit c (cursor
+// e = (c << sub_bits) | j) holds min(cursor[e], sub_cap) records from slot
+// seg_base(c, j). The passes after it read a coarse digit's sub-ranges as one
 // run of records (position p of digit c lies in the sub-range j with
 // segpre[c][j] <= p < segpre[c][j + 1]), cut into tiles as the dense layout
 // would be. k_seg_tiles writes, in `seg`: the first tile of every coarse
@@ -1126,7 +1132,7 @@ __device__ __forceinline__ int64_t seg_addr(const uint32_t* sp, int nsub, uint32
 	}
 	while (j + 1 < nsub && sp[j + 1] <= p)
 		++j;
-	return (int64_t)((c << sub_bits) | (uint32_t)j) * sub_cap + (p - sp[j]);
+	return seg_base(c, (uint32_t)j, sub_bits, sub_cap) + (p - sp[j]);
 }
 
 // The tile of workgroup i in the coarse-digit tiling (ts[0..256] in LDS):
@@ -1150,30 +1156,178 @@ __device__ __forceinline__ bool coarse_tile(const uint32_t* ts, int64_t i, int64
 	return true;
 }
 
-// The sub-range layout copied into one dense array, digits in order: the
-// exact regrouping after a bucket outgrew its region starts from it.
+// The digit holding coarse tile `tile` (ts: first tile of each digit, in
+// LDS) and the tile's index inside the digit
+__device__ __forceinline__ uint32_t tile_digit(const uint32_t* ts, uint32_t tile, uint32_t* k_out) {
+	int lo = 0, hi = RS_BINS - 1;  // largest c with ts[c] <= tile
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (ts[mid] <= tile)
+			lo = mid;
+		else
+			hi = mid - 1;
+	}
+	*k_out = tile - ts[lo];
+	return (uint32_t)lo;
+}
+
+// A wave's keys from the sub-range layout: positions [p0, p0 + IT * WAVE)
+// of digit c (round r lane l at p0 + r * WAVE + l), the first w_valid of
+// them present. They mostly lie in one or two sub-ranges: the first one and
+// the next two boundaries, found once for the wave, place every position
+// without further LDS reads; with more boundaries inside the range each lane
+// walks them. The loads carry the non-temporal hint (read once).
+template <int IT>
+__device__ __forceinline__ void seg_wave_keys(const uint32_t* sp, int nsub, uint32_t c, int sub_bits,
+                                              int64_t sub_cap, const uint64_t* __restrict__ in, int64_t p0,
+                                              int w_valid, uint64_t (&k)[IT]) {
+	const int lane = lane_id();
+	int j0 = -1;
+	(void)seg_addr(sp, nsub, c, sub_bits, sub_cap, (uint32_t)p0, j0);
+	const uint32_t b0 = sp[j0];
+	const uint32_t b1 = j0 + 1 < nsub ? sp[j0 + 1] : 0xFFFFFFFFu;
+	const uint32_t b2 = j0 + 2 < nsub ? sp[j0 + 2] : 0xFFFFFFFFu;
+	const uint32_t b3 = j0 + 3 < nsub ? sp[j0 + 3] : 0xFFFFFFFFu;
+	const bool few = (int64_t)b3 >= p0 + (int64_t)IT * WAVE;
+	int j = j0;
+#pragma unroll
+	for (int r = 0; r < IT; ++r) {
+		const bool valid = r * WAVE + lane < w_valid;
+		const uint32_t p = (uint32_t)(p0 + (int64_t)r * WAVE + lane);
+		int64_t at;
+		if (few) {
+			const int jj = j0 + (p >= b1 ? 1 : 0) + (p >= b2 ? 1 : 0);
+			const uint32_t bb = p >= b2 ? b2 : p >= b1 ? b1 : b0;
+			at = seg_base(c, (uint32_t)jj, sub_bits, sub_cap) + (p - bb);
+		} else {
+			at = valid ? seg_addr(sp, nsub, c, sub_bits, sub_cap, p, j) : 0;
+		}
+		k[r] = valid ? __builtin_nontemporal_load(&in[at]) : 0ull;
+	}
+}
+
+// The sub-range layout's coarse tiles [t0, t0 + ntiles) (the digits of one
+// part; ntiles < 0: every tile from t0 on) copied densely, digits in order,
+// the range's first digit at out[0]: the exact regrouping after a bucket
+// outgrew its region, or a chunk sorted in place, starts from it. The grid
+// covers the tile count.
 __global__ __launch_bounds__(256) void k_seg_compact(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                                                      const uint32_t* __restrict__ seg, int sub_bits,
-                                                     int64_t sub_cap) {
+                                                     int64_t sub_cap, int64_t t0, int64_t ntiles) {
 	__shared__ uint32_t s_ts[RS_BINS + 1];
 	__shared__ uint32_t s_sp[SEG_MAX_SUB + 1];
 	const int nsub = 1 << sub_bits;
 	for (int i = threadIdx.x; i <= RS_BINS; i += 256)
 		s_ts[i] = seg[SEG_TSTART + i];
 	__syncthreads();
-	uint32_t c, k;
-	if (!coarse_tile(s_ts, blockIdx.x, gridDim.x, &c, &k))  // uniform
+	const int64_t row = xcd_tile(blockIdx.x, gridDim.x);
+	if (row >= (ntiles < 0 ? (int64_t)s_ts[RS_BINS] - t0 : ntiles))  // uniform
 		return;
+	uint32_t k;
+	const uint32_t c = tile_digit(s_ts, (uint32_t)(t0 + row), &k);
+	uint32_t k0;
+	const uint32_t c0 = tile_digit(s_ts, (uint32_t)t0, &k0);
 	for (int i = threadIdx.x; i <= nsub; i += 256)
 		s_sp[i] = seg[SEG_PRE + (int64_t)c * seg_stride(nsub) + i];
 	__syncthreads();
 	const uint32_t total = s_sp[nsub];
 	const uint32_t p0 = k * RS_TILE;
 	const uint32_t n = total - p0 < (uint32_t)RS_TILE ? total - p0 : (uint32_t)RS_TILE;
-	const int64_t dst = (int64_t)seg[SEG_CSTART + c] + p0;
+	const int64_t dst = (int64_t)seg[SEG_CSTART + c] - (int64_t)seg[SEG_CSTART + c0] + p0;
 	int j = -1;
 	for (uint32_t i = threadIdx.x; i < n; i += 256)
 		out[dst + i] = in[seg_addr(s_sp, nsub, c, sub_bits, sub_cap, p0 + i, j)];
+}
+
+// The first grouping pass of a chunk whose records sit in the sub-range
+// layout: the chunk is coarse tiles [t0, t0 + ntiles) (the digits of its
+// part). k_seg_hist writes tile t0 + b's row b of counts of the digit at
+// `shift`; k_seg_scatter places the records densely by that digit at the
+// rows' scanned bases.
+__global__ __launch_bounds__(RS_NT) void k_seg_hist(const uint64_t* __restrict__ in, const uint32_t* __restrict__ seg,
+                                                    int64_t t0, int sub_bits, int64_t sub_cap, int shift,
+                                                    uint32_t* __restrict__ hist) {
+	constexpr int SEG = RS_IT * WAVE;
+	__shared__ uint32_t s_ts[RS_BINS + 1];
+	__shared__ uint32_t s_sp[SEG_MAX_SUB + 1];
+	__shared__ uint32_t s_h[RS_WAVES][RS_BINS];
+	const int t = threadIdx.x, w = t / WAVE;
+	const int nsub = 1 << sub_bits;
+	for (int i = t; i <= RS_BINS; i += RS_NT)
+		s_ts[i] = seg[SEG_TSTART + i];
+	for (int i = t; i < RS_WAVES * RS_BINS; i += RS_NT)
+		(&s_h[0][0])[i] = 0;
+	__syncthreads();
+	uint32_t kt;
+	const uint32_t c = tile_digit(s_ts, (uint32_t)(t0 + blockIdx.x), &kt);
+	for (int i = t; i <= nsub; i += RS_NT)
+		s_sp[i] = seg[SEG_PRE + (int64_t)c * seg_stride(nsub) + i];
+	__syncthreads();
+	const int64_t base = (int64_t)kt * RS_TILE;
+	const int64_t total = s_sp[nsub];
+	const int tile_n = (int)(total - base < RS_TILE ? total - base : RS_TILE);  // >= 1
+	const int wrem = tile_n - w * SEG;
+	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? wrem : SEG);
+	uint64_t k[RS_IT];
+	seg_wave_keys<RS_IT>(s_sp, nsub, c, sub_bits, sub_cap, in, base + (int64_t)w * SEG, w_valid, k);
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r)
+		if (r * WAVE + lane_id() < w_valid)
+			atomicAdd(&s_h[w][(uint32_t)(k[r] >> shift) & 0xFFu], 1u);
+	__syncthreads();
+	uint32_t cnt = 0;
+#pragma unroll
+	for (int ww = 0; ww < RS_WAVES; ++ww)
+		cnt += s_h[ww][t];
+	hist[(int64_t)blockIdx.x * RS_BINS + t] = cnt;
+}
+
+__global__ __launch_bounds__(RS_NT) void k_seg_scatter(const uint64_t* __restrict__ in,
+                                                       const uint32_t* __restrict__ seg, int64_t t0,
+                                                       int64_t ntiles, int sub_bits, int64_t sub_cap, int shift,
+                                                       const uint32_t* __restrict__ offs,
+                                                       uint64_t* __restrict__ out, int64_t n) {
+	constexpr int SEG = RS_IT * WAVE;
+	__shared__ uint64_t s_keys[RS_TILE];
+	__shared__ uint32_t s_cnt[RS_WAVES][RS_BINS];
+	__shared__ uint32_t s_dpre[RS_BINS + 1];
+	__shared__ uint32_t s_gbase[RS_BINS];
+	__shared__ uint32_t s_scan[RS_WAVES + 1];
+	__shared__ uint32_t s_ts[RS_BINS + 1];
+	__shared__ uint32_t s_sp[SEG_MAX_SUB + 1];
+	const int t = threadIdx.x, w = t / WAVE;
+	const int nsub = 1 << sub_bits;
+	for (int i = t; i <= RS_BINS; i += RS_NT)
+		s_ts[i] = seg[SEG_TSTART + i];
+	__syncthreads();
+	const int64_t row = xcd_tile(blockIdx.x, ntiles);
+	uint32_t kt;
+	const uint32_t c = tile_digit(s_ts, (uint32_t)(t0 + row), &kt);
+	for (int i = t; i <= nsub; i += RS_NT)
+		s_sp[i] = seg[SEG_PRE + (int64_t)c * seg_stride(nsub) + i];
+	s_gbase[t] = offs[row * RS_BINS + t];
+	__syncthreads();
+	const int64_t base = (int64_t)kt * RS_TILE;
+	const int64_t total = s_sp[nsub];
+	const int tile_n = (int)(total - base < RS_TILE ? total - base : RS_TILE);  // >= 1
+	const int wrem = tile_n - w * SEG;
+	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? wrem : SEG);
+	uint64_t k[RS_IT];
+	seg_wave_keys<RS_IT>(s_sp, nsub, c, sub_bits, sub_cap, in, base + (int64_t)w * SEG, w_valid, k);
+	uint32_t lp[RS_IT];  // order inside a digit run is free: the unstable rank
+	rank_tile_u<RS_NT, RS_IT>(k, w_valid, shift, lp, s_cnt, s_dpre, s_scan);
+#pragma unroll
+	for (int r = 0; r < RS_IT; ++r)
+		if (lp[r] != 0xFFFFFFFFu)
+			s_keys[lp[r]] = k[r];
+	__syncthreads();
+	for (int i = t; i < tile_n; i += RS_NT) {
+		const uint64_t key = s_keys[i];
+		const uint32_t d = (uint32_t)(key >> shift) & 0xFFu;
+		const uint32_t g = s_gbase[d] + ((uint32_t)i - s_dpre[d]);
+		if ((int64_t)g < n)
+			out[g] = key;
+	}
 }
 
 struct RegionArgs {
@@ -1233,32 +1387,7 @@ __global__ __launch_bounds__(NT) void k_region_scatter(RegionArgs a) {
 	const int w_valid = wrem <= 0 ? 0 : (wrem < SEG ? wrem : SEG);
 	uint64_t k[IT];
 	if constexpr (SEGS) {
-		// the wave's positions [seg, seg + SEG) mostly lie in one or two
-		// sub-ranges: the first one and the next two boundaries, found once
-		// for the wave, place every position without further LDS reads;
-		// with more boundaries inside the range each lane walks them
-		int j0 = -1;
-		(void)seg_addr(s_sp, nsub, c, a.sub_bits, a.sub_cap, (uint32_t)seg, j0);
-		const uint32_t b0 = s_sp[j0];
-		const uint32_t b1 = j0 + 1 < nsub ? s_sp[j0 + 1] : 0xFFFFFFFFu;
-		const uint32_t b2 = j0 + 2 < nsub ? s_sp[j0 + 2] : 0xFFFFFFFFu;
-		const uint32_t b3 = j0 + 3 < nsub ? s_sp[j0 + 3] : 0xFFFFFFFFu;
-		const bool few = (int64_t)b3 >= seg + SEG;
-		int j = j0;
-#pragma unroll
-		for (int r = 0; r < IT; ++r) {
-			const bool valid = r * WAVE + lane < w_valid;
-			const uint32_t p = (uint32_t)(seg + (int64_t)r * WAVE + lane);
-			int64_t at;
-			if (few) {
-				const int jj = j0 + (p >= b1 ? 1 : 0) + (p >= b2 ? 1 : 0);
-				const uint32_t bb = p >= b2 ? b2 : p >= b1 ? b1 : b0;
-				at = (int64_t)((c << a.sub_bits) | (uint32_t)jj) * a.sub_cap + (p - bb);
-			} else {
-				at = valid ? seg_addr(s_sp, nsub, c, a.sub_bits, a.sub_cap, p, j) : 0;
-			}
-			k[r] = valid ? __builtin_nontemporal_load(&a.in[at]) : 0ull;  // read once
-		}
+		seg_wave_keys<IT>(s_sp, nsub, c, a.sub_bits, a.sub_cap, a.in, seg, w_valid, k);
 	} else {
 #pragma unroll
 		for (int r = 0; r < IT; ++r) {

@@ -786,6 +786,8 @@ struct Grouping {
 	bool coarse = false;  // region layout whose coarse pass the generator does (segment layout in recs_alt)
 	const uint64_t* src = nullptr;  // region layout: the chunk's records read from here, not from recs
 	                                // (its slice of the part-ordered records, left intact for later rounds)
+	int64_t seg_t0 = -1, seg_nt = 0;  // region layout: the chunk's records are coarse tiles
+	                                  // [seg_t0, seg_t0 + seg_nt) of the generator's sub-ranges (recs_all)
 	int nseg = 0;
 	int64_t sub_cap = 0;
 	int64_t rtiles = 0;
@@ -835,16 +837,35 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 		StageScope sg(h, "sort");
 		uint32_t* hist = h->hist.as<uint32_t>();
 		const uint64_t* in = g.src ? g.src : h->recs.as<uint64_t>();
-		if (!g.fused_hist) {
-			ProfScope ps(h, "sort_rec_hist", (double)g.nrec * 8.0);
-			k_radix_hist<<<(unsigned)g.rtiles, RS_NT, 0, s>>>(in, g.nrec, g.hi - 8, hist, g.rtiles);
+		if (g.seg_t0 >= 0) {
+			// the part's digits read from the generator's sub-ranges
+			TRY(ensure(h, h->hist, sizeof(uint32_t) * (size_t)(RS_BINS * g.seg_nt)));
+			hist = h->hist.as<uint32_t>();
+			{
+				ProfScope ps(h, "sort_rec_hist", (double)g.nrec * 8.0);
+				k_seg_hist<<<(unsigned)g.seg_nt, RS_NT, 0, s>>>(h->recs_all.as<uint64_t>(), h->seg_ctl.as<uint32_t>(),
+				                                                g.seg_t0, COARSE_SUB_BITS, g.sub_cap, g.hi - 8, hist);
+				HIP_LAUNCH_CHECK(h);
+			}
+			uint32_t* dtot = nullptr;
+			TRY(digit_scan(h, hist, g.seg_nt, &dtot));
+			k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
 			HIP_LAUNCH_CHECK(h);
-		}
-		uint32_t* dtot = nullptr;
-		TRY(digit_scan(h, hist, g.rtiles, &dtot));
-		k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
-		HIP_LAUNCH_CHECK(h);
-		{
+			ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)g.nrec);
+			k_seg_scatter<<<(unsigned)g.seg_nt, RS_NT, 0, s>>>(h->recs_all.as<uint64_t>(), h->seg_ctl.as<uint32_t>(),
+			                                                   g.seg_t0, g.seg_nt, COARSE_SUB_BITS, g.sub_cap,
+			                                                   g.hi - 8, hist, h->recs_alt.as<uint64_t>(), g.nrec);
+			HIP_LAUNCH_CHECK(h);
+		} else {
+			if (!g.fused_hist) {
+				ProfScope ps(h, "sort_rec_hist", (double)g.nrec * 8.0);
+				k_radix_hist<<<(unsigned)g.rtiles, RS_NT, 0, s>>>(in, g.nrec, g.hi - 8, hist, g.rtiles);
+				HIP_LAUNCH_CHECK(h);
+			}
+			uint32_t* dtot = nullptr;
+			TRY(digit_scan(h, hist, g.rtiles, &dtot));
+			k_coarse_tiles<<<1, RS_BINS, 0, s>>>(dtot, h->ctl.as<uint32_t>());
+			HIP_LAUNCH_CHECK(h);
 			ProfScope ps(h, "sort_rec_scatter", 16.0 * (double)g.nrec);
 			k_radix_scatter<false, RS_NT, true><<<(unsigned)g.rtiles, RS_NT, 0, s>>>(
 			    in, nullptr, h->recs_alt.as<uint64_t>(), nullptr, g.nrec, g.hi - 8, hist, g.rtiles);
@@ -892,13 +913,15 @@ int regroup_exact(xtn_handle* h, Grouping& g) {
 	if (g.coarse) {
 		// the generator's segment layout, compacted into recs
 		k_seg_compact<<<(unsigned)(g.rtiles + RS_BINS), 256, 0, h->stream>>>(
-		    h->recs_alt.as<uint64_t>(), h->recs.as<uint64_t>(), h->seg_ctl.as<uint32_t>(), COARSE_SUB_BITS, g.sub_cap);
+		    h->recs_alt.as<uint64_t>(), h->recs.as<uint64_t>(), h->seg_ctl.as<uint32_t>(), COARSE_SUB_BITS, g.sub_cap,
+		    0, -1);
 		HIP_LAUNCH_CHECK(h);
 		g.coarse = false;
 	} else {
 		std::swap(h->recs, h->recs_alt);
 	}
 	g.src = nullptr;
+	g.seg_t0 = -1;
 	g.region = false;
 	g.fused_hist = false;
 	return issue_grouping(h, g);
@@ -1137,7 +1160,63 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	// all V records again to keep its own
 	const bool gen_once = nparts > 1 && pbits <= 8 && h->cfg.max_records_per_chunk <= 0 && !h->no_alloc &&
 	                      (double)V * 16.0 + (double)(V >> pbits) * 1.25 * 40.0 <= (double)free_b * 0.6;
-	if (gen_once) {
+	// generated once, through the generator's coarse pass when it applies:
+	// the records land in sub-ranges of their top 8 hash bits, and each part
+	// (a range of those digits) is read from there by its first pass
+	bool gen_seg = false;
+	std::vector<int64_t> part_t0(nparts + 1, 0);
+	int64_t seg_sub_cap = 0;
+	if (gen_once && h->cfg.coarse_separate == 0 && !h->coarse_off) {
+		TRY(prologue());
+		const int nseg = RS_BINS << COARSE_SUB_BITS;
+		const double mean = (double)V / nseg;
+		seg_sub_cap = (int64_t)(mean * 1.1 + 8.0 * std::sqrt(mean) + 64.0) | 1;
+		TRY(ensure(h, h->recs_all, sizeof(uint64_t) * (size_t)(seg_sub_cap * nseg)));
+		TRY(ensure(h, h->seg_cur, sizeof(uint32_t) * (size_t)nseg));
+		TRY(ensure(h, h->seg_ctl, sizeof(uint32_t) * (size_t)seg_words(1 << COARSE_SUB_BITS)));
+		TRY(set_ctr(h, offsetof(RunCtrs, coarse_ovf), 0ull));
+		k_fill_u64<<<grid_for(nseg / 2, 256), 256, 0, s>>>(h->seg_cur.as<unsigned long long>(), nseg / 2, 0ull);
+		HIP_LAUNCH_CHECK(h);
+		ga.out = h->recs_all.as<uint64_t>();
+		ga.cap = seg_sub_cap * nseg;
+		ga.digit_hist = nullptr;
+		ga.digit_shift = 56;
+		ga.sub_bits = COARSE_SUB_BITS;
+		ga.cursor = h->seg_cur.as<uint32_t>();
+		ga.coarse_ovf = ctr_field(h, offsetof(RunCtrs, coarse_ovf));
+		ga.sub_cap = seg_sub_cap;
+		{
+			StageScope sg(h, "generate");
+			ProfScope ps(h, "gen_coarse", (double)V * 8.0 + (double)nseq * 24.0);
+			k_gen<GEN_COARSE, GEN_COARSE_NT><<<gen_grid, GEN_COARSE_NT, 0, s>>>(ga);
+			HIP_LAUNCH_CHECK(h);
+		}
+		k_seg_tiles<1 << COARSE_SUB_BITS><<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), seg_sub_cap,
+		                                                       h->seg_ctl.as<uint32_t>(), nullptr, 0);
+		HIP_LAUNCH_CHECK(h);
+		std::vector<uint32_t> segh((size_t)seg_words(1 << COARSE_SUB_BITS));
+		HIP_TRY(h, hipMemcpyAsync(segh.data(), h->seg_ctl.p, sizeof(uint32_t) * segh.size(), hipMemcpyDeviceToHost, s));
+		RunCtrs rc{};
+		TRY(read_ctrs(h, &rc));
+		if (rc.coarse_ovf) {
+			h->coarse_off = true;  // a sub-range overflowed: the separate pass below
+		} else {
+			gen_seg = true;
+			const int per = RS_BINS >> pbits;  // top-8-bit digits per part
+			const int nsub = 1 << COARSE_SUB_BITS;
+			cap_recs = 0;
+			for (int p = 0; p < nparts; ++p) {
+				unsigned long long c = 0;
+				for (int k = 0; k < per; ++k)
+					c += segh[SEG_PRE + (size_t)(p * per + k) * seg_stride(nsub) + nsub];
+				part_count[p] = c;
+				part_t0[p] = segh[SEG_TSTART + p * per];
+				cap_recs = std::max<int64_t>(cap_recs, (int64_t)c);
+			}
+			part_t0[nparts] = segh[SEG_TSTART + RS_BINS];
+		}
+	}
+	if (gen_once && !gen_seg) {
 		TRY(prologue());
 		const int64_t vt = ((int64_t)V + RS_TILE - 1) / RS_TILE;
 		TRY(ensure(h, h->recs_all, sizeof(uint64_t) * (size_t)V));
@@ -1179,6 +1258,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		cap_recs = 0;
 		for (auto c : part_count)
 			cap_recs = std::max<int64_t>(cap_recs, (int64_t)c);
+	} else if (gen_seg) {
+		// part counts and tile ranges read back above
 	} else if (nparts > 1) {
 		TRY(prologue());
 		TRY(ensure(h, h->part_hist, sizeof(unsigned long long) * nparts));
@@ -1246,7 +1327,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		if (g.coarse) {
 			g.nseg = RS_BINS << COARSE_SUB_BITS;
 			const double mean = (double)nrec / g.nseg;
-			g.sub_cap = (int64_t)(mean * 1.1 + 8.0 * std::sqrt(mean) + 64.0);
+			g.sub_cap = (int64_t)(mean * 1.1 + 8.0 * std::sqrt(mean) + 64.0) | 1;
 			TRY(ensure(h, h->recs_alt, sizeof(uint64_t) * (size_t)(g.sub_cap * g.nseg)));
 			TRY(ensure(h, h->seg_cur, sizeof(uint32_t) * (size_t)g.nseg));
 			TRY(ensure(h, h->seg_ctl, sizeof(uint32_t) * (size_t)seg_words(1 << COARSE_SUB_BITS)));
@@ -1303,6 +1384,20 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ProfScope ps(h, "gen", (double)V * 8.0 + (double)nseq * 24.0 + (g.fused_hist ? 1024.0 * g.rtiles : 0.0));
 			k_gen<GEN_DIRECT><<<gen_grid, GEN_NT, 0, s>>>(ga);
 			HIP_LAUNCH_CHECK(h);
+		} else if (gen_seg && g.region) {
+			// the region layout's first pass reads the part's digits from the sub-ranges
+			g.seg_t0 = part_t0[part];
+			g.seg_nt = part_t0[part + 1] - part_t0[part];
+			g.sub_cap = seg_sub_cap;
+		} else if (gen_seg) {
+			ProfScope ps(h, "part_copy", 16.0 * (double)part_count[part]);
+			const int64_t nt = part_t0[part + 1] - part_t0[part];
+			if (nt > 0) {
+				k_seg_compact<<<(unsigned)nt, 256, 0, s>>>(h->recs_all.as<uint64_t>(), h->recs.as<uint64_t>(),
+				                                           h->seg_ctl.as<uint32_t>(), COARSE_SUB_BITS, seg_sub_cap,
+				                                           part_t0[part], nt);
+				HIP_LAUNCH_CHECK(h);
+			}
 		} else if (gen_once && g.region) {
 			// the region layout's first pass reads the part's slice in place
 			g.src = h->recs_part.as<uint64_t>() + part_start[part];
@@ -1542,7 +1637,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		++st.n_rounds;
 		st.tail_checked = h->tail_safe ? 1 : 0;
 		st.tail_passes = h->tail_safe ? (2 * idb + 7) / 8 : tail_passes(h, idb);
-		st.coarse_in_generator = used_coarse ? 1 : 0;
+		st.coarse_in_generator = used_coarse || gen_seg ? 1 : 0;
 		rounds.pop_front();
 	}
 	h->last_pairs = max_round;
