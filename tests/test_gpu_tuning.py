@@ -112,18 +112,21 @@ def test_partial_sort_in_batched_sets_and_rounds(low):
 def test_chunks_cut_by_bucket_target():
     # 37M records over a 512-record bucket target exceed the 65536 buckets a
     # chunk may hold, so the run splits into chunks by bucket size: the
-    # records are generated once and each chunk copies its part; a chunk size
-    # set by the caller generates every part on its own. Both match the oracle,
+    # records are generated once and each chunk reads its part; a chunk size
+    # set by the caller generates every part on its own. All match the oracle,
     # over one round and over several.
     chars, offsets = xtn.synth_nn(54, 300_000, planted=0.25, max_edits=2)
     want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
     with xtn.Engine(distance=2) as eng:
         eng.set_sequences(chars=chars, offsets=offsets)
         eng.set_tuning(bucket_capacity=2048, bucket_target=512)
-        for chunk, pairs in ((0, 0), (0, 1 << 16), (12_000_000, 0)):
-            eng.configure(max_records_per_chunk=chunk, max_pairs_per_round=pairs)
+        # generated once through the coarse pass (one round and several), once
+        # with the separate pass, and per part
+        for chunk, pairs, sep in ((0, 0, 0), (0, 1 << 16, 0), (0, 0, 1), (12_000_000, 0, 0)):
+            eng.configure(max_records_per_chunk=chunk, max_pairs_per_round=pairs, coarse_separate=sep)
             n = eng.nn_resident()
-            assert np.array_equal(triples(*eng.copy_nn_output(n)), want), (chunk, pairs)
+            assert np.array_equal(triples(*eng.copy_nn_output(n)), want), (chunk, pairs, sep)
             st = eng.stats()
             assert st["n_chunks"] > 1
             assert (st["n_rounds"] > 1) == (pairs > 0)
+            assert st["coarse_in_generator"] == (1 if chunk == 0 and sep == 0 else 0)
