@@ -192,6 +192,9 @@ typedef struct {
 	int32_t candidates;       /* xtn_autotune: settings timed */
 	int32_t sort_low_bits;    /* candidate bits below which the sort stops, ranked inside each
 	                             group by the select: 0 (default: full sort), 8, 16 or 24 */
+	int32_t coarse_separate;  /* 1: the records' first ordering pass runs as a pass of its own
+	                             (as xtn_config.coarse_separate); 0 (default): inside the generator */
+	int32_t reserved;
 } xtn_tuning;
 
 int xtn_get_tuning(xtn_handle* h, xtn_tuning* out);
@@ -203,7 +206,7 @@ int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t);
  * two warm-up runs) as xtn_run_nn_resident, its output checked against the
  * default setting's; the search goes coordinate by coordinate (bucket
  * capacity and target, then the sort tile, then the select tile, then the
- * sort's low bits) and keeps
+ * sort's low bits, then the generator's coarse pass on or off) and keeps
  * the fastest, the default unless another beats it by more than 1% when
  * both are timed again. With cache_path, the choice is read from that JSON
  * file when an entry for the workload (sequence count, distance, measure,

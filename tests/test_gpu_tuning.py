@@ -42,11 +42,29 @@ def test_settings_match_oracle(cap, target, nt, tile, sel, low):
         assert eng.tuning()["sort_low_bits"] == low
 
 
+def test_tuned_separate_pass_matches_oracle():
+    # the tuned switch sends the records' first ordering pass out of the
+    # generator as the config's does, on one chunk and on chunks cut by
+    # bucket size, and back in when it is cleared
+    chars, offsets = xtn.synth_nn(53, 300_000, planted=0.25, max_edits=2)
+    want = oracle.nn_port_mt(chars, offsets, 2, 0, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=2) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        for cap, target, sep in ((3072, 2200, 1), (2048, 512, 1), (2048, 512, 0), (3072, 2200, 0)):
+            eng.set_tuning(bucket_capacity=cap, bucket_target=target, coarse_separate=sep)
+            for _ in range(2):
+                n = eng.nn_resident()
+                assert np.array_equal(triples(*eng.copy_nn_output(n)), want), (target, sep)
+                assert eng.stats()["coarse_in_generator"] == 1 - sep
+            assert eng.tuning()["coarse_separate"] == sep
+
+
 def test_invalid_settings_rejected():
     with xtn.Engine(distance=1) as eng:
         for kw in (dict(bucket_capacity=1000), dict(sort_threads=128), dict(bucket_capacity=2048, bucket_target=2000),
                    dict(bucket_capacity=4096, bucket_target=500), dict(sort_threads=256, sort_tile=8192),
-                   dict(sort_threads=1024, sort_tile=4096), dict(select_threads=1024), dict(sort_low_bits=12)):
+                   dict(sort_threads=1024, sort_tile=4096), dict(select_threads=1024), dict(sort_low_bits=12),
+                   dict(coarse_separate=2)):
             with pytest.raises(xtn.XtnError) as e:
                 eng.set_tuning(**kw)
             assert e.value.status == 3
@@ -70,7 +88,8 @@ def test_autotune_and_cache(tmp_path):
         eng.set_sequences(chars=chars, offsets=offsets)
         t2 = eng.autotune(cache, runs=2)
         assert t2["from_cache"] == 1
-        keys = ("bucket_capacity", "bucket_target", "sort_threads", "sort_tile", "select_threads", "sort_low_bits")
+        keys = ("bucket_capacity", "bucket_target", "sort_threads", "sort_tile", "select_threads", "sort_low_bits",
+                "coarse_separate")
         assert [t2[k] for k in keys] == [t[k] for k in keys]
         n = eng.nn_resident()
         assert np.array_equal(triples(*eng.copy_nn_output(n)), want)

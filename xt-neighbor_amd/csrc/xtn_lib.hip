@@ -121,6 +121,7 @@ struct xtn_handle {
 	int tune_sort_tile = 4096;  // keys of a pair-sort tile: 4096 or 8192
 	int tune_select_nt = 256;   // threads of a select tile (8 candidates each): 256 or 512
 	int tune_low = 0;           // pair bits left unsorted for the select's in-group ranking: 0, 8, 16, 24
+	int tune_coarse_sep = 0;    // 1: the records' first ordering pass as a pass of its own (tuned)
 	bool tail_full = false;     // a group outgrew the select's window: full sorts until new sequences
 	bool coarse_off = false;    // a sub-range of the generator's coarse pass overflowed: the separate
 	                            // pass until new sequences
@@ -1166,7 +1167,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 	bool gen_seg = false;
 	std::vector<int64_t> part_t0(nparts + 1, 0);
 	int64_t seg_sub_cap = 0;
-	if (gen_once && h->cfg.coarse_separate == 0 && !h->coarse_off) {
+	if (gen_once && h->cfg.coarse_separate == 0 && h->tune_coarse_sep == 0 && !h->coarse_off) {
 		TRY(prologue());
 		const int nseg = RS_BINS << COARSE_SUB_BITS;
 		const double mean = (double)V / nseg;
@@ -1323,7 +1324,8 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		// sub_cap slots (the mean with 10 % and eight standard deviations of
 		// headroom; a sub-range that still overflows sends the run back to the
 		// separate pass)
-		g.coarse = g.region && nparts == 1 && !h->coarse_off && !h->tail_safe && h->cfg.coarse_separate == 0;
+		g.coarse = g.region && nparts == 1 && !h->coarse_off && !h->tail_safe && h->cfg.coarse_separate == 0 &&
+		           h->tune_coarse_sep == 0;
 		if (g.coarse) {
 			g.nseg = RS_BINS << COARSE_SUB_BITS;
 			const double mean = (double)nrec / g.nseg;
@@ -1488,7 +1490,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 				sig.sort_tile = h->tune_sort_tile;
 				sig.select_nt = h->tune_select_nt;
 				sig.low = h->tail_full ? 0 : h->tune_low;
-				sig.coarse = h->coarse_off || h->cfg.coarse_separate ? 0 : 1;
+				sig.coarse = h->coarse_off || h->cfg.coarse_separate || h->tune_coarse_sep ? 0 : 1;
 				sig.pad = 0;
 			}
 			bool via_graph = false;
@@ -2327,6 +2329,7 @@ int xtn_get_tuning(xtn_handle* h, xtn_tuning* out) {
 	out->sort_tile = h->tune_sort_tile;
 	out->select_threads = h->tune_select_nt;
 	out->sort_low_bits = h->tune_low;
+	out->coarse_separate = h->tune_coarse_sep;
 	out->ms = h->tune_ms;
 	out->ms_default = h->tune_ms_default;
 	return XTN_OK;
@@ -2341,17 +2344,19 @@ int xtn_set_tuning(xtn_handle* h, const xtn_tuning* t) {
 	const int lb = t->sort_low_bits;
 	if ((c != 2048 && c != 3072 && c != 4096) || !sort_ok ||
 	    (t->select_threads != 256 && t->select_threads != 512) || t->bucket_target < c / 4 ||
-	    t->bucket_target > c * 9 / 10 || (lb != 0 && lb != 8 && lb != 16 && lb != 24))
+	    t->bucket_target > c * 9 / 10 || (lb != 0 && lb != 8 && lb != 16 && lb != 24) ||
+	    (t->coarse_separate != 0 && t->coarse_separate != 1))
 		return fail(h, XTN_ERR_ARG,
 		            "invalid tuning (capacity %d, target %d, sort threads %d, sort tile %d, select threads %d, "
-		            "low bits %d)",
-		            c, t->bucket_target, nt, tile, t->select_threads, lb);
+		            "low bits %d, separate coarse pass %d)",
+		            c, t->bucket_target, nt, tile, t->select_threads, lb, t->coarse_separate);
 	h->tune_cap = c;
 	h->tune_target = t->bucket_target;
 	h->tune_sort_nt = nt;
 	h->tune_sort_tile = tile;
 	h->tune_select_nt = t->select_threads;
 	h->tune_low = lb;
+	h->tune_coarse_sep = t->coarse_separate;
 	return XTN_OK;
 }
 
@@ -2433,7 +2438,7 @@ static bool tune_cache_read(const char* path, const std::string& sig, xtn_tuning
 			const char* p = strstr(line, name);
 			return p && sscanf(p + strlen(name), " : %lf", v) == 1;
 		};
-		double c, tg, nt, tl, sn, lb, ms, msd;
+		double c, tg, nt, tl, sn, lb, ms, msd, cs = 0;
 		if (num("\"bucket_capacity\"", &c) && num("\"bucket_target\"", &tg) && num("\"sort_threads\"", &nt) &&
 		    num("\"sort_tile\"", &tl) && num("\"select_threads\"", &sn) && num("\"sort_low_bits\"", &lb) &&
 		    num("\"ms\"", &ms) && num("\"ms_default\"", &msd)) {
@@ -2443,6 +2448,7 @@ static bool tune_cache_read(const char* path, const std::string& sig, xtn_tuning
 			t->sort_tile = (int32_t)tl;
 			t->select_threads = (int32_t)sn;
 			t->sort_low_bits = (int32_t)lb;
+			t->coarse_separate = num("\"coarse_separate\"", &cs) ? (int32_t)cs : 0;  // absent: older entries
 			t->ms = ms;
 			t->ms_default = msd;
 			found = true;
@@ -2469,10 +2475,10 @@ static bool tune_cache_write(const char* path, const std::string& sig, const xtn
 	char buf[512];
 	snprintf(buf, sizeof(buf),
 	         "  {\"workload\": \"%s\", \"bucket_capacity\": %d, \"bucket_target\": %d, \"sort_threads\": %d, "
-	         "\"sort_tile\": %d, \"select_threads\": %d, \"sort_low_bits\": %d, \"ms\": %.4f, "
-	         "\"ms_default\": %.4f}",
+	         "\"sort_tile\": %d, \"select_threads\": %d, \"sort_low_bits\": %d, \"coarse_separate\": %d, "
+	         "\"ms\": %.4f, \"ms_default\": %.4f}",
 	         sig.c_str(), t.bucket_capacity, t.bucket_target, t.sort_threads, t.sort_tile, t.select_threads,
-	         t.sort_low_bits, t.ms, t.ms_default);
+	         t.sort_low_bits, t.coarse_separate, t.ms, t.ms_default);
 	keep.push_back(buf);
 	const std::string tmp = std::string(path) + ".tmp";
 	FILE* f = fopen(tmp.c_str(), "w");
@@ -2535,10 +2541,10 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 	// its output must be the default setting's
 	int32_t runs_now = runs;
 	struct Setting {
-		int cap, target, sort_nt, sort_tile, select_nt, low;
+		int cap, target, sort_nt, sort_tile, select_nt, low, sep;
 		bool operator==(const Setting& o) const {
 			return cap == o.cap && target == o.target && sort_nt == o.sort_nt && sort_tile == o.sort_tile &&
-			       select_nt == o.select_nt && low == o.low;
+			       select_nt == o.select_nt && low == o.low && sep == o.sep;
 		}
 	};
 	uint64_t dig_def = 0;
@@ -2551,6 +2557,7 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 		h->tune_sort_tile = st.sort_tile;
 		h->tune_select_nt = st.select_nt;
 		h->tune_low = st.low;
+		h->tune_coarse_sep = st.sep;
 		std::vector<double> t;
 		for (int r = 0; r < runs_now + 2; ++r) {
 			memset(&h->stats, 0, sizeof(h->stats));
@@ -2567,40 +2574,46 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 			have_def = true;
 		} else if (dig != dig_def) {
 			return fail(h, XTN_ERR_HIP,
-			            "autotune: capacity %d, target %d, sort %dx%d, select %d, low bits %d changed the output",
-			            st.cap, st.target, st.sort_nt, st.sort_tile, st.select_nt, st.low);
+			            "autotune: capacity %d, target %d, sort %dx%d, select %d, low bits %d, separate coarse pass %d "
+			            "changed the output",
+			            st.cap, st.target, st.sort_nt, st.sort_tile, st.select_nt, st.low, st.sep);
 		}
 		++tried;
 		return XTN_OK;
 	};
-	const Setting def{3072, 2200, 512, 4096, 256, 0};
+	const Setting def{3072, 2200, 512, 4096, 256, 0, 0};
 	double ms_def = 0;
 	TRY(time_setting(def, &ms_def));
 	Setting cur = def;
 	double cur_ms = ms_def;
 	// coordinate descent: bucket capacity and target, then the sort tile,
-	// then the select tile, then the sort's low bits, each step keeping the
-	// best so far
+	// then the select tile, then the sort's low bits, then the generator's
+	// coarse pass on or off, each step keeping the best so far
 	const int caps[3] = {3072, 4096, 2048};
 	const int targets[3][3] = {{1840, 2200, 2600}, {2450, 2950, 3480}, {1230, 1470, 1740}};
 	std::vector<Setting> step1;
 	for (int ci = 0; ci < 3; ++ci)
 		for (int tg : targets[ci])
-			step1.push_back({caps[ci], tg, def.sort_nt, def.sort_tile, def.select_nt, def.low});
-	for (int step = 1; step <= 4; ++step) {
+			step1.push_back({caps[ci], tg, def.sort_nt, def.sort_tile, def.select_nt, def.low, def.sep});
+	for (int step = 1; step <= 5; ++step) {
 		std::vector<Setting> cands = step1;
 		const Setting c = cur;
 		if (step == 2)
-			cands = {{c.cap, c.target, 256, 4096, c.select_nt, c.low}, {c.cap, c.target, 512, 4096, c.select_nt, c.low},
-			         {c.cap, c.target, 512, 8192, c.select_nt, c.low}, {c.cap, c.target, 1024, 8192, c.select_nt, c.low}};
+			cands = {{c.cap, c.target, 256, 4096, c.select_nt, c.low, c.sep},
+			         {c.cap, c.target, 512, 4096, c.select_nt, c.low, c.sep},
+			         {c.cap, c.target, 512, 8192, c.select_nt, c.low, c.sep},
+			         {c.cap, c.target, 1024, 8192, c.select_nt, c.low, c.sep}};
 		if (step == 3)
-			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, 256, c.low},
-			         {c.cap, c.target, c.sort_nt, c.sort_tile, 512, c.low}};
+			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, 256, c.low, c.sep},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, 512, c.low, c.sep}};
 		if (step == 4)
-			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 0},
-			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 8},
-			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 16},
-			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 24}};
+			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 0, c.sep},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 8, c.sep},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 16, c.sep},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, 24, c.sep}};
+		if (step == 5)
+			cands = {{c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, c.low, 0},
+			         {c.cap, c.target, c.sort_nt, c.sort_tile, c.select_nt, c.low, 1}};
 		for (const Setting& st : cands) {
 			if (st == cur)
 				continue;
@@ -2636,6 +2649,7 @@ int xtn_autotune(xtn_handle* h, const char* cache_path, int32_t runs, xtn_tuning
 	best.sort_tile = cur.sort_tile;
 	best.select_threads = cur.select_nt;
 	best.sort_low_bits = cur.low;
+	best.coarse_separate = cur.sep;
 	best.ms = cur_ms;
 	best.ms_default = ms_def;
 	best.candidates = tried;

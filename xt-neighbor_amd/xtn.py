@@ -99,7 +99,8 @@ class Tuning(ctypes.Structure):
                 ("sort_threads", ctypes.c_int32), ("sort_tile", ctypes.c_int32),
                 ("select_threads", ctypes.c_int32), ("from_cache", ctypes.c_int32),
                 ("ms", ctypes.c_double), ("ms_default", ctypes.c_double),
-                ("candidates", ctypes.c_int32), ("sort_low_bits", ctypes.c_int32)]
+                ("candidates", ctypes.c_int32), ("sort_low_bits", ctypes.c_int32),
+                ("coarse_separate", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(Output))
@@ -497,9 +498,11 @@ class Engine:
         return {name: getattr(t, name) for name, _ in Tuning._fields_}
 
     def set_tuning(self, bucket_capacity: int = 3072, bucket_target: int = 2200, sort_threads: int = 512,
-                   sort_tile: int = 4096, select_threads: int = 256, sort_low_bits: int = 0):
+                   sort_tile: int = 4096, select_threads: int = 256, sort_low_bits: int = 0,
+                   coarse_separate: int = 0):
         t = Tuning(bucket_capacity=bucket_capacity, bucket_target=bucket_target, sort_threads=sort_threads,
-                   sort_tile=sort_tile, select_threads=select_threads, sort_low_bits=sort_low_bits)
+                   sort_tile=sort_tile, select_threads=select_threads, sort_low_bits=sort_low_bits,
+                   coarse_separate=coarse_separate)
         self._check(self.lib.xtn_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, cache_path: str | None = None, runs: int = 5) -> dict:
