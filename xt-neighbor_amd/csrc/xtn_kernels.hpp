@@ -485,7 +485,9 @@ __device__ __forceinline__ void seq_hash_table(const GenArgs& a, int64_t sid, ui
 	tab[MAX_PREFIX + 1] = c;
 }
 
-template <int MODE, int NT = GEN_NT>
+// SMALL: only distances up to 3 (no unranking of larger deletion sets, whose
+// registers would then set the whole kernel's budget)
+template <int MODE, int NT = GEN_NT, bool SMALL = false>
 __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 	constexpr int IT = GEN_TILE / NT;  // records per lane
 	// variant offsets of the tile's sequences minus r0, clamped to
@@ -507,10 +509,14 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 	} s_v;
 	uint16_t* const s_vrel = s_v.vrel;
 	__shared__ int32_t s_v0;
-	__shared__ union {
+	union GenU {
 		GenTables tab;              // d >= 4
 		uint32_t del[DEL_SETS3];    // d <= 3
-	} s_u;
+	};
+	struct GenSmallU {
+		uint32_t del[DEL_SETS3];
+	};
+	__shared__ std::conditional_t<SMALL, GenSmallU, GenU> s_u;
 	__shared__ u128 s_hm[MAX_PREFIX + 1], s_lm[MAX_PREFIX + 1];
 	__shared__ int64_t s_lo;
 	__shared__ int s_cnt;
@@ -523,14 +529,15 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 		return;
 	const int64_t r1 = r0 + GEN_TILE < a.V ? r0 + GEN_TILE : a.V;
 
-	const bool small_d = a.distance <= 3;
+	const bool small_d = SMALL || a.distance <= 3;
 	if (small_d) {
 		const uint32_t* src = a.distance == 1 ? c_del_tables.d1 : a.distance == 2 ? c_del_tables.d2 : c_del_tables.d3;
 		const int ns = a.distance == 1 ? DEL_SETS1 : a.distance == 2 ? DEL_SETS2 : DEL_SETS3;
 		for (int i = t; i < ns; i += NT)
 			s_u.del[i] = src[i];
 	} else {
-		load_gen_tables(&s_u.tab, t, NT);
+		if constexpr (!SMALL)
+			load_gen_tables(&s_u.tab, t, NT);
 	}
 	init_slot_masks(s_hm, s_lm, t, NT);
 	if (MODE == GEN_DIRECT || MODE == GEN_COARSE)
@@ -692,9 +699,11 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 							v = del_slot_t(v, (int)((e >> 8) & 0xFFu), s_hm, s_lm);
 						v = del_slot_t(v, (int)(e & 0xFFu), s_hm, s_lm);
 					} else {
-						const int L = (int)(phi >> 56);
-						const int m = a.distance < L ? a.distance : L;
-						v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
+						if constexpr (!SMALL) {
+							const int L = (int)(phi >> 56);
+							const int m = a.distance < L ? a.distance : L;
+							v = unrank_variant(p, L, m, k, s_u.tab.binom, s_hm, s_lm, s_u.tab.pair, s_u.tab.triple);
+						}
 					}
 					// the set bits make variants of different query sets different keys
 					if (poly)

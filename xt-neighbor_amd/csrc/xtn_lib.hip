@@ -1018,6 +1018,15 @@ int checked_candidates(xtn_handle* h, Grouping& g, uint32_t lo, uint32_t hi, int
 // the round through checked_candidates. Candidates beyond the round budget
 // (pmax) split the smaller-id range into rounds whose outputs follow each
 // other in order (the reference's lower-bound rounds, xtn.cu:385-470).
+// the generator with the records' first ordering pass; distances up to 3 take
+// the instantiation without the unranking of larger deletion sets
+void launch_gen_coarse(const GenArgs& ga, unsigned grid, hipStream_t s) {
+	if (ga.distance <= 3)
+		k_gen<GEN_COARSE, GEN_COARSE_NT, true><<<grid, GEN_COARSE_NT, 0, s>>>(ga);
+	else
+		k_gen<GEN_COARSE, GEN_COARSE_NT><<<grid, GEN_COARSE_NT, 0, s>>>(ga);
+}
+
 int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t row_base = 0,
                 const uint32_t* set_of = nullptr) {
 	hipStream_t s = h->stream;
@@ -1189,7 +1198,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		{
 			StageScope sg(h, "generate");
 			ProfScope ps(h, "gen_coarse", (double)V * 8.0 + (double)nseq * 24.0);
-			k_gen<GEN_COARSE, GEN_COARSE_NT><<<gen_grid, GEN_COARSE_NT, 0, s>>>(ga);
+			launch_gen_coarse(ga, gen_grid, s);
 			HIP_LAUNCH_CHECK(h);
 		}
 		k_seg_tiles<1 << COARSE_SUB_BITS><<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), seg_sub_cap,
@@ -1379,7 +1388,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			ga.coarse_ovf = ctr_field(h, offsetof(RunCtrs, coarse_ovf));
 			ga.sub_cap = g.sub_cap;
 			ProfScope ps(h, "gen_coarse", (double)V * 8.0 + (double)nseq * 24.0);
-			k_gen<GEN_COARSE, GEN_COARSE_NT><<<gen_grid, GEN_COARSE_NT, 0, s>>>(ga);
+			launch_gen_coarse(ga, gen_grid, s);
 			HIP_LAUNCH_CHECK(h);
 			used_coarse = true;
 		} else if (nparts == 1) {
