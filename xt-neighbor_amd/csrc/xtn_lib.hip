@@ -1136,7 +1136,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		if (v_known && nparts == 1 && !h->tail_safe) {
 			// packing, offsets and tile table in one launch (its look-back
 			// shares the error flag the single-chunk run reads at its end)
-			const int64_t nb = (nseq + PK_NT - 1) / PK_NT;
+			const int64_t nb = (nseq + PK_SEQ - 1) / PK_SEQ;
 			const uint64_t ntiles = (V + GEN_TILE - 1) / GEN_TILE;
 			TRY(ensure(h, h->pack_state, sizeof(uint64_t) * (size_t)nb));
 			TRY(ensure(h, h->tseq, sizeof(uint32_t) * (size_t)(ntiles + 1)));

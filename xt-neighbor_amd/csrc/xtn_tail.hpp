@@ -760,7 +760,8 @@ inline SelectFn select_kernel(int distance, int measure, int nt, bool grouped) {
 
 // k_pack, the exclusive scan of the variant counts into voff and the
 // generator's tile table in one launch: workgroup b packs sequences
-// [PK_NT b, PK_NT b + PK_NT), scans their counts, and adds the counts of the
+// [PK_SEQ b, PK_SEQ b + PK_SEQ), two consecutive ones per thread (so that one
+// round of workgroups covers a million sequences), scans their counts, and adds the counts of the
 // workgroups before it by a wave-wide decoupled look-back over 64-bit status
 // words (flag | 62-bit sum; state[] cleared before the launch). Each sequence
 // then writes its offset and records the generator tiles whose first record
@@ -769,7 +770,9 @@ inline SelectFn select_kernel(int distance, int measure, int nt, bool grouped) {
 // has at least one variant, itself). The waits are on lower-numbered
 // workgroups, which each XCD dispatches first; a wait past spin_limit
 // raises *error and the host redoes the run.
-constexpr int PK_NT = 1024;  // sequences per workgroup: a short look-back chain
+constexpr int PK_NT = 1024;
+constexpr int PK_PER = 2;                // consecutive sequences per thread
+constexpr int PK_SEQ = PK_NT * PK_PER;  // sequences per workgroup: a short look-back chain
 
 __global__ __launch_bounds__(PK_NT) void k_pack_scan(const uint8_t* __restrict__ chars,
                                                    const uint32_t* __restrict__ offs,
@@ -781,10 +784,17 @@ __global__ __launch_bounds__(PK_NT) void k_pack_scan(const uint8_t* __restrict__
 	__shared__ uint32_t s_tmp[PK_NT / WAVE + 1];
 	__shared__ uint64_t s_prefix;
 	const int t = threadIdx.x;
-	const int32_t i = blockIdx.x * PK_NT + t;
-	const uint32_t c = i < n ? pack_one(chars, offs, seq_map, row_base, set_of, i, distance, pk) : 0u;
+	const int32_t i0 = blockIdx.x * PK_SEQ + PK_PER * t;
+	uint32_t c[PK_PER];
+	uint32_t csum = 0;
+#pragma unroll
+	for (int u = 0; u < PK_PER; ++u) {
+		const int32_t i = i0 + u;
+		c[u] = i < n ? pack_one(chars, offs, seq_map, row_base, set_of, i, distance, pk) : 0u;
+		csum += c[u];
+	}
 	uint32_t tot;
-	const uint32_t ex = block_excl_scan<PK_NT>(c, s_tmp, &tot);  // <= PK_NT * 2^18
+	const uint32_t ex = block_excl_scan<PK_NT>(csum, s_tmp, &tot);  // <= PK_SEQ * 2^18
 	const int64_t b = blockIdx.x;
 	if (t < WAVE) {
 		if (t == 0)
@@ -797,15 +807,21 @@ __global__ __launch_bounds__(PK_NT) void k_pack_scan(const uint8_t* __restrict__
 		}
 	}
 	__syncthreads();
-	if (i >= n)
-		return;
-	const uint64_t o = s_prefix + ex, e = o + c;
-	voff[i] = o;
-	for (uint64_t bt = (o + GEN_TILE - 1) / GEN_TILE; bt * GEN_TILE < e; ++bt)
-		tile_seq[bt] = (uint32_t)i;
-	if (i == n - 1) {
-		voff[n] = e;
-		tile_seq[(e + GEN_TILE - 1) / GEN_TILE] = (uint32_t)i;
+	uint64_t o = s_prefix + ex;
+#pragma unroll
+	for (int u = 0; u < PK_PER; ++u) {
+		const int32_t i = i0 + u;
+		if (i >= n)
+			return;
+		const uint64_t e = o + c[u];
+		voff[i] = o;
+		for (uint64_t bt = (o + GEN_TILE - 1) / GEN_TILE; bt * GEN_TILE < e; ++bt)
+			tile_seq[bt] = (uint32_t)i;
+		if (i == n - 1) {
+			voff[n] = e;
+			tile_seq[(e + GEN_TILE - 1) / GEN_TILE] = (uint32_t)i;
+		}
+		o = e;
 	}
 }
 
