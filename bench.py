@@ -365,8 +365,8 @@ def autotuned(eng, enabled: bool) -> dict:
     t = eng.autotune(TUNE_CACHE, runs=5)
     return {"enabled": True, "bucket_capacity": t["bucket_capacity"], "bucket_target": t["bucket_target"],
             "sort_threads": t["sort_threads"], "sort_tile": t["sort_tile"], "select_threads": t["select_threads"],
-            "sort_low_bits": t["sort_low_bits"], "ms_tuned": t["ms"], "ms_default": t["ms_default"],
-            "candidates": t["candidates"]}
+            "sort_low_bits": t["sort_low_bits"], "coarse_separate": t["coarse_separate"], "ms_tuned": t["ms"],
+            "ms_default": t["ms_default"], "candidates": t["candidates"]}
 
 
 def timed_runs(eng, steps):
