@@ -487,6 +487,31 @@ __device__ __forceinline__ void seq_hash_table(const GenArgs& a, int64_t sid, ui
 
 // SMALL: only distances up to 3 (no unranking of larger deletion sets, whose
 // registers would then set the whole kernel's budget)
+// Positions q < L of a packed prefix whose residue equals the one before it.
+// Deleting any position of a run of equal residues leaves the same string, so
+// a deletion set D gives the same variant as a set that deletes, in every
+// run, a prefix of the run; only those sets need a record:
+// D is kept when no q in D has its bit here while q - 1 is not in D.
+__device__ __forceinline__ uint32_t run_mask(uint64_t lo, uint64_t hw) {
+	const uint32_t hi26 = (uint32_t)hw & PK_SLOT_HI_MASK;
+	const int L = (int)(hw >> 56);
+	uint32_t m = 0, prev = 0xFFFFFFFFu;
+#pragma unroll
+	for (int q = 0; q < MAX_PREFIX; ++q) {
+		const uint32_t c = slot_of(lo, hi26, q);
+		if (q < L && c == prev)
+			m |= 1u << q;
+		prev = c;
+	}
+	return m;
+}
+
+__device__ __forceinline__ bool dup_deletion_set(uint32_t e, uint32_t runs) {
+	const uint32_t dm = ((1u << (e & 0xFFu)) | (1u << ((e >> 8) & 0xFFu)) | (1u << (e >> 16))) &
+	                    ((1u << MAX_PREFIX) - 1u);  // NO_DEL (18) drops out
+	return (dm & runs & ~(dm << 1)) != 0u;
+}
+
 template <int MODE, int NT = GEN_NT, bool SMALL = false>
 __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 	constexpr int IT = GEN_TILE / NT;  // records per lane
@@ -522,6 +547,7 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 	__shared__ int s_cnt;
 	__shared__ uint32_t s_hist[MODE == GEN_HIST ? 4096 : 1];
 	__shared__ uint32_t s_dig[MODE == GEN_DIRECT ? RS_BINS : MODE == GEN_COARSE ? RS_BINS + 1 : 1];
+	__shared__ uint32_t s_run[MODE == GEN_COARSE ? GEN_TBL : 1];  // run_mask of the tile's sequences (form 0)
 
 	const int t = threadIdx.x;
 	const int64_t r0 = (int64_t)blockIdx.x * GEN_TILE;
@@ -585,8 +611,11 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 	const bool poly = poly_keys(a.distance);
 	const int form = poly && cnt <= GEN_TBL ? 0 : cnt <= GEN_PKL ? 1 : 2;
 	if (form == 0)
-		for (int i = t; i < cnt; i += NT)
+		for (int i = t; i < cnt; i += NT) {
 			seq_hash_table(a, slo + i, s_v.t.tab[i]);
+			if constexpr (MODE == GEN_COARSE)
+				s_run[i] = run_mask(a.pk[2 * (slo + i)], a.pk[2 * (slo + i) + 1]);
+		}
 	else if (form == 1)
 		for (int i = t; i < cnt; i += NT) {
 			s_v.p.pk[i][0] = a.pk[2 * (slo + i)];
@@ -598,6 +627,7 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 	const uint64_t idmask = (1ull << a.idb) - 1ull;
 	uint64_t kept[MODE == GEN_APPEND || MODE == GEN_COARSE ? IT : 1];
 	uint32_t nkept = 0;
+	uint32_t dropm = 0;  // GEN_COARSE: bit it set when record it repeats a variant of its sequence
 	// The lane's first record: the sequence holding it (largest idx with
 	// s_vrel[idx] <= t). Every iteration then finds the next record's
 	// sequence before generating its own. Forms 0 and 1 read nothing from
@@ -665,12 +695,15 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 				}
 			}
 			uint64_t rec = 0;
+			bool dup = false;
 			if (valid) {
 				const int64_t sid = slo + idx;
 				const uint32_t k = (uint32_t)(rr - start);
 				uint64_t hv;
 				if constexpr (TBL) {  // the deletion set's table terms
 					const uint32_t e = s_u.del[k];
+					if constexpr (MODE == GEN_COARSE)
+						dup = dup_deletion_set(e, s_run[idx]);
 					const uint64_t* tb = s_v.t.tab[idx];
 					// every byte of an entry is <= NO_DEL: the three reads go out together
 					const uint64_t t1 = tb[e & 0xFFu], t2 = tb[(e >> 8) & 0xFFu], t3 = tb[e >> 16];
@@ -725,8 +758,10 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 			} else if (MODE == GEN_COARSE) {
 				// the record stays in a register; lanes past the end count
 				// into the spare bin s_dig[RS_BINS]
+				// a repeated variant counts into the spare bin too and is not written
 				kept[it] = rec;
-				atomicAdd(&s_dig[valid ? (uint32_t)(rec >> a.digit_shift) & 0xFFu : (uint32_t)RS_BINS], 1u);
+				dropm |= (uint32_t)dup << it;
+				atomicAdd(&s_dig[valid && !dup ? (uint32_t)(rec >> a.digit_shift) & 0xFFu : (uint32_t)RS_BINS], 1u);
 			} else {
 				const bool mine = valid && (a.part_bits == 0 || (uint32_t)(rec >> (64 - a.part_bits)) == a.part_id);
 				if (mine)
@@ -777,7 +812,8 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 		__shared__ uint32_t s_tmp[NT / WAVE + 1];
 		static_assert(sizeof(s_u) >= (3 * RS_BINS + 1) * sizeof(uint32_t), "offsets fit the generator's tables");
 		const uint32_t c = t < RS_BINS ? s_dig[t] : 0u;
-		const uint32_t ex = block_excl_scan<NT>(c, s_tmp, (uint32_t*)nullptr);
+		uint32_t n_kept;  // records written: those of the 256 digits
+		const uint32_t ex = block_excl_scan<NT>(c, s_tmp, &n_kept);
 		const uint32_t sub = blockIdx.x & ((1u << a.sub_bits) - 1u);
 		if (t < RS_BINS) {
 			s_dpre[t] = ex;
@@ -792,13 +828,13 @@ __global__ __launch_bounds__(NT) void k_gen(GenArgs a) {
 		uint32_t rank[IT];  // the record's position in the tile sorted by digit
 #pragma unroll
 		for (int r = 0; r < IT; ++r) {
-			const bool valid = r0 + r * NT + t < r1;
+			const bool valid = r0 + r * NT + t < r1 && !((dropm >> r) & 1u);
 			rank[r] = atomicAdd(&s_cur[valid ? (uint32_t)(kept[r] >> a.digit_shift) & 0xFFu : (uint32_t)RS_BINS], 1u);
 		}
 		// the sorted tile in two halves, so that LDS holds half of it (more
 		// workgroups per CU for the generation): a record goes to the half
 		// its position falls in; the digit runs keep their length
-		const int tile_n = (int)(r1 - r0);
+		const int tile_n = (int)n_kept;
 		constexpr uint32_t HALF = GEN_TILE / 2;
 		for (uint32_t lo_pos = 0; lo_pos < (uint32_t)GEN_TILE; lo_pos += HALF) {
 #pragma unroll

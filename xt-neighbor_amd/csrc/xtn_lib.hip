@@ -912,11 +912,26 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 // bucket bit and bucketed by bounds
 int regroup_exact(xtn_handle* h, Grouping& g) {
 	if (g.coarse) {
-		// the generator's segment layout, compacted into recs
+		// the generator's segment layout, compacted into recs; it holds the
+		// digits' totals rather than nrec records (the generator writes no
+		// record for a repeated variant of a sequence). The caller has just
+		// synchronised: the totals are read back here
+		const int nsub = 1 << COARSE_SUB_BITS;
+		std::vector<uint32_t> ctl((size_t)seg_words(nsub));
+		HIP_TRY(h, hipMemcpyAsync(ctl.data(), h->seg_ctl.p, sizeof(uint32_t) * ctl.size(), hipMemcpyDeviceToHost,
+		                          h->stream));
+		HIP_SYNC(h, h->stream);
+		int64_t total = 0;
+		for (int c = 0; c < RS_BINS; ++c)
+			total += ctl[(size_t)SEG_PRE + (size_t)c * seg_stride(nsub) + (size_t)nsub];
+		if (total > g.nrec)
+			return fail(h, XTN_ERROR, "sub-range totals %lld exceed the %lld records generated", (long long)total,
+			            (long long)g.nrec);
 		k_seg_compact<<<(unsigned)(g.rtiles + RS_BINS), 256, 0, h->stream>>>(
 		    h->recs_alt.as<uint64_t>(), h->recs.as<uint64_t>(), h->seg_ctl.as<uint32_t>(), COARSE_SUB_BITS, g.sub_cap,
 		    0, -1);
 		HIP_LAUNCH_CHECK(h);
+		g.nrec = total;
 		g.coarse = false;
 	} else {
 		std::swap(h->recs, h->recs_alt);
