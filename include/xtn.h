@@ -110,7 +110,10 @@ typedef struct {
 	int64_t n_variants;    /* deletion records generated (V) */
 	int64_t n_chunks;      /* key-space chunks */
 	int64_t n_runs;        /* key runs holding >= 2 records */
-	int64_t n_candidates;  /* candidate pairs i != j emitted from runs */
+	int64_t n_candidates;  /* candidate pairs i != j emitted from runs (with
+	                          multiplicity; the copies a sequence's repeated
+	                          variants would add are not there when the
+	                          generator's coarse pass left them out) */
 	int64_t n_unique;      /* unique candidate pairs verified (U) */
 	int64_t n_output;      /* pairs (NN) or repertoire entries (overlap) out */
 	double ms_total;       /* device pipeline wall time */
@@ -132,6 +135,11 @@ typedef struct {
 	                                (always after a sub-range of the generator's
 	                                pass overflowed on this handle, until new
 	                                sequences) */
+	int64_t n_records;     /* deletion records written: n_variants less the
+	                          repeated variants of a sequence (deleting any
+	                          residue of a run of equal ones gives the same
+	                          string), which the generator's coarse pass
+	                          leaves out; n_variants on the separate pass */
 } xtn_stats;
 
 /* fills *cfg with defaults: distance 1, Levenshtein, device 0 */

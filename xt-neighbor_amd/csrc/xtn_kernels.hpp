@@ -45,6 +45,7 @@ struct RunCtrs {
 	unsigned long long n_unique;     // unique candidate pairs verified
 	unsigned long long append;       // partition append cursor
 	unsigned long long coarse_ovf;   // the generator's coarse pass lost records (sub-range full)
+	unsigned long long n_records;    // records the generator's coarse pass wrote (k_seg_tiles)
 };
 
 struct Run {
@@ -1087,7 +1088,7 @@ constexpr int seg_words(int nsub) { return SEG_PRE + RS_BINS * seg_stride(nsub);
 template <int NSUB>
 __global__ __launch_bounds__(SEG_NT) void k_seg_tiles(const uint32_t* __restrict__ cursor, int64_t sub_cap,
                                                       uint32_t* __restrict__ seg, uint32_t* __restrict__ bzero,
-                                                      int64_t nzero) {
+                                                      int64_t nzero, unsigned long long* __restrict__ n_rec) {
 	constexpr int PER = RS_BINS * NSUB / SEG_NT;  // a thread's segments, all of one digit
 	constexpr int GPT = NSUB / PER;               // threads per digit
 	static_assert(PER % 4 == 0 && NSUB % PER == 0 && NSUB <= SEG_MAX_SUB, "whole digits, u32x4 rows");
@@ -1147,6 +1148,8 @@ __global__ __launch_bounds__(SEG_NT) void k_seg_tiles(const uint32_t* __restrict
 	uint32_t all_c, all_t;
 	const uint32_t cs = block_excl_scan<SEG_NT>(ct, s_tmp, &all_c);
 	const uint32_t ts = block_excl_scan<SEG_NT>((ct + RS_TILE - 1) / RS_TILE, s_tmp, &all_t);
+	if (t == 0 && n_rec)
+		*n_rec = all_c;  // the records in the layout
 	if (t < RS_BINS) {
 		seg[SEG_TSTART + t] = ts;
 		seg[SEG_CSTART + t] = cs;

@@ -811,7 +811,8 @@ int issue_grouping(xtn_handle* h, Grouping& g) {
 	if (g.region && g.coarse) {
 		StageScope sg(h, "sort");
 		k_seg_tiles<1 << COARSE_SUB_BITS><<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), g.sub_cap,
-		                                                       h->seg_ctl.as<uint32_t>(), h->bcount.as<uint32_t>(), g.nb);
+		                                                       h->seg_ctl.as<uint32_t>(), h->bcount.as<uint32_t>(), g.nb,
+		                                                       ctr_field(h, offsetof(RunCtrs, n_records)));
 		HIP_LAUNCH_CHECK(h);
 		RegionArgs ra{};
 		ra.in = h->recs_alt.as<uint64_t>();
@@ -1217,7 +1218,7 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 			HIP_LAUNCH_CHECK(h);
 		}
 		k_seg_tiles<1 << COARSE_SUB_BITS><<<1, SEG_NT, 0, s>>>(h->seg_cur.as<uint32_t>(), seg_sub_cap,
-		                                                       h->seg_ctl.as<uint32_t>(), nullptr, 0);
+		                                                       h->seg_ctl.as<uint32_t>(), nullptr, 0, nullptr);
 		HIP_LAUNCH_CHECK(h);
 		std::vector<uint32_t> segh((size_t)seg_words(1 << COARSE_SUB_BITS));
 		HIP_TRY(h, hipMemcpyAsync(segh.data(), h->seg_ctl.p, sizeof(uint32_t) * segh.size(), hipMemcpyDeviceToHost, s));
@@ -1665,6 +1666,21 @@ int nn_pipeline(xtn_handle* h, int32_t nseq, const uint32_t* seq_map, uint32_t r
 		st.tail_passes = h->tail_safe ? (2 * idb + 7) / 8 : tail_passes(h, idb);
 		st.coarse_in_generator = used_coarse || gen_seg ? 1 : 0;
 		rounds.pop_front();
+	}
+	// records written: fewer than V when the generator's coarse pass left
+	// out repeated variants (its total is in the counters read at the end)
+	st.n_records = (int64_t)V;
+	if (gen_seg) {
+		st.n_records = 0;
+		for (int p = 0; p < nparts; ++p)
+			st.n_records += (int64_t)part_count[p];
+	} else if (used_coarse) {
+		if (head.rc.n_records == 0) {  // no tail ran: the counters were not read back
+			RunCtrs rc{};
+			TRY(read_ctrs(h, &rc));
+			head.rc.n_records = rc.n_records;
+		}
+		st.n_records = (int64_t)head.rc.n_records;
 	}
 	h->last_pairs = max_round;
 	h->last_unique = st.n_unique;

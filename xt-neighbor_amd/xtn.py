@@ -91,7 +91,8 @@ class Stats(ctypes.Structure):
                 ("ms_output", ctypes.c_double), ("n_rounds", ctypes.c_int64),
                 ("tail_checked", ctypes.c_int64), ("n_launches", ctypes.c_int64),
                 ("n_syncs", ctypes.c_int64), ("graph_replays", ctypes.c_int64),
-                ("tail_passes", ctypes.c_int64), ("coarse_in_generator", ctypes.c_int64)]
+                ("tail_passes", ctypes.c_int64), ("coarse_in_generator", ctypes.c_int64),
+                ("n_records", ctypes.c_int64)]
 
 
 class Tuning(ctypes.Structure):
