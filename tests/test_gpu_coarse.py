@@ -105,3 +105,60 @@ def test_generator_pass_sweep(seed, n, d, measure, dup):
             got, st = run(eng, distance=d, measure=measure, coarse_separate=0)
             assert np.array_equal(got, want)
             assert st["coarse_in_generator"] == 1
+
+
+def runs_input(d, measure):
+    # strings made of runs (homopolymers, doubled residues) and of
+    # alternations (ABAB...), whose repeats the run test does not catch,
+    # mixed with ordinary rows and their planted relatives
+    rng = np.random.default_rng(80 + d + 10 * measure)
+    aa = "ACDEFGHIKLMNPQRSTVWY"
+    seqs = []
+    for _ in range(6000):
+        kind = rng.integers(0, 3)
+        n = int(rng.integers(4, 21))
+        if kind == 0:
+            seqs.append("".join(aa[int(rng.integers(0, 3))] * int(rng.integers(1, 5)) for _ in range(n // 2 + 1))[:n])
+        elif kind == 1:
+            a, b = aa[int(rng.integers(0, 20))], aa[int(rng.integers(0, 20))]
+            seqs.append("".join(a if i % 2 == 0 else b for i in range(n)))
+        else:
+            seqs.append("".join(c * 2 for c in rng.choice(list(aa), n // 2 + 1))[:n])
+    base = xtn.decode(*xtn.synth_nn(90 + d, 20_000, planted=0.3, max_edits=d))
+    seqs = seqs + base
+    seqs = [seqs[i] for i in rng.permutation(len(seqs))]
+    return xtn.encode(seqs)
+
+
+@pytest.mark.parametrize("d,measure", [(2, 0), (2, 1), (3, 0), (3, 1)])
+def test_runs_of_equal_residues(d, measure):
+    # the generator's pass leaves out most of these rows' deletion sets and
+    # the output is the oracle's
+    chars, offsets = runs_input(d, measure)
+    want = oracle.nn_port_mt(chars, offsets, d, measure, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=d) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        got, st = run(eng, distance=d, measure=measure, coarse_separate=0)
+        assert np.array_equal(got, want)
+        assert st["coarse_in_generator"] == 1
+        assert st["n_records"] < st["n_variants"]
+
+
+XFAIL_SEPARATE = pytest.mark.xfail(
+    reason="known: at d = 3 on this input the separate first pass, which writes every deletion set "
+           "(165 equal records of one id for one variant here), loses 7 of 139,647 unique candidates "
+           "and one output pair (rows 'AAAAACAAAADDAAC' / 'AAAAAAAAAAADDAAA'); cause not found yet",
+    strict=False)
+
+
+@pytest.mark.parametrize("d,measure", [(2, 0), (2, 1), pytest.param(3, 0, marks=XFAIL_SEPARATE),
+                                       pytest.param(3, 1, marks=XFAIL_SEPARATE)])
+def test_runs_of_equal_residues_separate_pass(d, measure):
+    chars, offsets = runs_input(d, measure)
+    want = oracle.nn_port_mt(chars, offsets, d, measure, THREADS)[0].astype(np.int64)
+    with xtn.Engine(distance=d) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        got, st = run(eng, distance=d, measure=measure, coarse_separate=1)
+        assert st["coarse_in_generator"] == 0
+        assert st["n_records"] == st["n_variants"]
+        assert np.array_equal(got, want)
