@@ -144,16 +144,11 @@ def test_runs_of_equal_residues(d, measure):
         assert st["n_records"] < st["n_variants"]
 
 
-XFAIL_SEPARATE = pytest.mark.xfail(
-    reason="known: at d = 3 on this input the separate first pass, which writes every deletion set "
-           "(165 equal records of one id for one variant here), loses 7 of 139,647 unique candidates "
-           "and one output pair (rows 'AAAAACAAAADDAAC' / 'AAAAAAAAAAADDAAA'); cause not found yet",
-    strict=False)
-
-
-@pytest.mark.parametrize("d,measure", [(2, 0), (2, 1), pytest.param(3, 0, marks=XFAIL_SEPARATE),
-                                       pytest.param(3, 1, marks=XFAIL_SEPARATE)])
+@pytest.mark.parametrize("d,measure", [(2, 0), (2, 1), (3, 0), (3, 1)])
 def test_runs_of_equal_residues_separate_pass(d, measure):
+    # every deletion set written (165 equal records of one id for one variant
+    # here): buckets outgrow the LDS and their long runs go through the
+    # flattened pair kernel, whose pairs were once filtered by the wrong range
     chars, offsets = runs_input(d, measure)
     want = oracle.nn_port_mt(chars, offsets, d, measure, THREADS)[0].astype(np.int64)
     with xtn.Engine(distance=d) as eng:

@@ -1724,22 +1724,24 @@ __global__ __launch_bounds__(256) void k_pairs_big(const uint64_t* __restrict__ 
 		bool ok = false;
 		uint64_t p = 0;
 		if (q < total) {
-			int64_t lo = 0, hi = nruns - 1;  // run with pre[run] <= q < pre[run+1]
-			while (lo < hi) {
-				int64_t mid = (lo + hi + 1) >> 1;
+			// the run with pre[run] <= q < pre[run+1] (its own names: lo and
+			// hi are the id range the pairs are kept for)
+			int64_t r_lo = 0, r_hi = nruns - 1;
+			while (r_lo < r_hi) {
+				int64_t mid = (r_lo + r_hi + 1) >> 1;
 				if (pre[mid] <= q)
-					lo = mid;
+					r_lo = mid;
 				else
-					hi = mid - 1;
+					r_hi = mid - 1;
 			}
-			const uint64_t u = q - pre[lo];
+			const uint64_t u = q - pre[r_lo];
 			uint64_t b = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)u)) * 0.5);
 			while (b * (b - 1) / 2 > u)
 				--b;
 			while ((b + 1) * b / 2 <= u)
 				++b;
 			const uint64_t a = u - b * (b - 1) / 2;
-			const uint32_t st = runs[lo].start;
+			const uint32_t st = runs[r_lo].start;
 			ok = pair_of(rec[st + a], rec[st + b], idb, lo, hi, &p);
 		}
 		const unsigned long long slot = wave_append(&ctr->n_cand, ok ? 1u : 0u);
