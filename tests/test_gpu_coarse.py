@@ -157,3 +157,36 @@ def test_runs_of_equal_residues_separate_pass(d, measure):
         assert st["coarse_in_generator"] == 0
         assert st["n_records"] == st["n_variants"]
         assert np.array_equal(got, want)
+
+
+@pytest.fixture(scope="module")
+def runs_d3():
+    chars, offsets = runs_input(3, 0)
+    return chars, offsets, oracle.nn_port_mt(chars, offsets, 3, 0, THREADS)[0].astype(np.int64)
+
+
+@pytest.mark.parametrize("name,tuning,cfg", [
+    ("chunks cut by memory", {}, dict(max_records_per_chunk=-3)),
+    ("rounds, separate pass", {}, dict(max_pairs_per_round=8 << 20, coarse_separate=1)),
+    ("small buckets, separate pass", dict(bucket_capacity=2048, bucket_target=1500), dict(coarse_separate=1)),
+    ("small buckets", dict(bucket_capacity=2048, bucket_target=1500), {}),
+])
+def test_runs_of_equal_residues_fallback_paths(runs_d3, name, tuning, cfg):
+    # the same rows through the paths that take over when buckets, rounds or
+    # memory overflow (long runs of equal keys reach the flattened pair kernel)
+    chars, offsets, want = runs_d3
+    with xtn.Engine(distance=3) as eng:
+        eng.set_sequences(chars=chars, offsets=offsets)
+        eng.nn_resident()
+        v = eng.stats()["n_variants"]
+        eng.set_tuning(**tuning)
+        c = dict(cfg)
+        if c.get("max_records_per_chunk") == -3:
+            c["max_records_per_chunk"] = v // 3
+        for _ in range(2):
+            got, st = run(eng, distance=3, measure=0, **c)
+            assert np.array_equal(got, want), (name, st)
+        if "max_records_per_chunk" in c:
+            assert st["n_chunks"] > 1
+        if "max_pairs_per_round" in c:
+            assert st["n_rounds"] > 1
